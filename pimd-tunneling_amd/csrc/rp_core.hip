@@ -1,0 +1,375 @@
+// rp_core.hip -- runtime, errors, and the potential (mcmod_mass) objects.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "rp_dispatch.h"
+
+namespace rp {
+
+static thread_local std::string g_err;
+static int g_device = -1;
+static std::mutex g_init_mu;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int ensure_init() {
+  if (g_device >= 0) {
+    RP_HIP(hipSetDevice(g_device));
+    return RP_OK;
+  }
+  return rp_init(0);
+}
+
+// ---------------------------------------------------------------- kernels
+// One thread per bead; x/grad in [dof][bead] layout with leading dimension ld.
+template <int KIND, int MAXD>
+__global__ void __launch_bounds__(128) pes_eval_kernel(PesDev P, int nbead, int ld,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ grad,
+                                                       double* __restrict__ energy) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbead) return;
+  double xs[MAXD], gs[MAXD];
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d) xs[d] = (d < P.ndof) ? x[(size_t)d * ld + b] : 0.0;
+  double v = eval_kind<KIND, MAXD>(P, xs, gs);
+  if (energy) energy[b] = v;
+  if (grad) {
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d)
+      if (d < P.ndof) grad[(size_t)d * ld + b] = gs[d];
+  }
+}
+
+template <int KIND, int MAXD>
+__global__ void __launch_bounds__(128) pes_hvp_kernel(PesDev P, int nbead, int ld,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ v,
+                                                      double* __restrict__ hv) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbead) return;
+  Dual xs[MAXD], gs[MAXD];
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d)
+    xs[d] = (d < P.ndof) ? Dual(x[(size_t)d * ld + b], v[(size_t)d * ld + b]) : Dual(0.0);
+  (void)eval_kind<KIND, MAXD>(P, xs, gs);
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d)
+    if (d < P.ndof) hv[(size_t)d * ld + b] = gs[d].d;
+}
+
+// Hessian rows of one bead: hess[b][i][k] = d grad_k / d x_i from ndof HVPs with e_i.
+template <int KIND, int MAXD>
+__global__ void __launch_bounds__(64) pes_hess_kernel(PesDev P, int nbead,
+                                                      const double* __restrict__ x,
+                                                      double* __restrict__ hess) {
+  int b = blockIdx.x;
+  int i = threadIdx.x;
+  if (b >= nbead || i >= P.ndof) return;
+  Dual xs[MAXD], gs[MAXD];
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d)
+    xs[d] = (d < P.ndof) ? Dual(x[(size_t)d * nbead + b], d == i ? 1.0 : 0.0) : Dual(0.0);
+  (void)eval_kind<KIND, MAXD>(P, xs, gs);
+  const int D = P.ndof;
+#pragma unroll
+  for (int k = 0; k < MAXD; ++k)
+    if (k < D) hess[((size_t)b * D + i) * D + k] = gs[k].d;
+}
+
+// p -= h * grad V(x), chains [chain][dof][bead] with n beads each.
+template <int KIND, int MAXD>
+__global__ void __launch_bounds__(128) pes_kick_kernel(PesDev P, int n, int nchain,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ p, double h,
+                                                       int* __restrict__ nan_flag) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * nchain) return;
+  int c = t / n, b = t - c * n;
+  const size_t base = (size_t)c * P.ndof * n + b;
+  double xs[MAXD], gs[MAXD];
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d) xs[d] = (d < P.ndof) ? x[base + (size_t)d * n] : 0.0;
+  (void)eval_kind<KIND, MAXD>(P, xs, gs);
+  bool bad = false;
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d) {
+    if (d < P.ndof) {
+      double np = p[base + (size_t)d * n] - gs[d] * h;
+      p[base + (size_t)d * n] = np;
+      bad |= (np != np);
+    }
+  }
+  if (bad) atomicOr(nan_flag, 1);
+}
+
+template <int NMOL>
+__global__ void __launch_bounds__(128) water_param_grad_kernel(PesDev P, int nbead,
+                                                               const double* __restrict__ x,
+                                                               double* __restrict__ out) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbead) return;
+  double xs[9 * NMOL], gs[9 * NMOL], pg[2];
+#pragma unroll
+  for (int d = 0; d < 9 * NMOL; ++d) xs[d] = x[(size_t)d * nbead + b];
+  (void)water_eval<NMOL>(P, xs, gs, pg);
+  out[2 * b] = pg[0];
+  out[2 * b + 1] = pg[1];
+}
+
+int launch_pes_eval(const rp_pes* pes, int nbead, int ld, const double* x, double* grad,
+                    double* energy, hipStream_t s) {
+  if (nbead <= 0) return RP_OK;
+  PesDev P = pes->dev();
+  return dispatch_pes(pes, [&]<int K, int M>() -> int {
+    pes_eval_kernel<K, M><<<(nbead + 127) / 128, 128, 0, s>>>(P, nbead, ld, x, grad, energy);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  });
+}
+
+int launch_pes_hvp(const rp_pes* pes, int nbead, int ld, const double* x, const double* v,
+                   double* hv, hipStream_t s) {
+  if (nbead <= 0) return RP_OK;
+  PesDev P = pes->dev();
+  return dispatch_pes(pes, [&]<int K, int M>() -> int {
+    pes_hvp_kernel<K, M><<<(nbead + 127) / 128, 128, 0, s>>>(P, nbead, ld, x, v, hv);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  });
+}
+
+int launch_pes_kick(const rp_pes* pes, int n, int nchain, const double* x, double* p, double h,
+                    int* nan_flag, hipStream_t s) {
+  int tot = n * nchain;
+  if (tot <= 0) return RP_OK;
+  PesDev P = pes->dev();
+  return dispatch_pes(pes, [&]<int K, int M>() -> int {
+    pes_kick_kernel<K, M><<<(tot + 127) / 128, 128, 0, s>>>(P, n, nchain, x, p, h, nan_flag);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  });
+}
+
+int pes_maxd(const rp_pes* pes) {
+  int D = pes->ndof;
+  if (pes->kind == PES_WATER) return D;
+  if (D <= 16) return 16;
+  if (D <= 32) return 32;
+  if (D <= 64) return 64;
+  return 0;
+}
+
+}  // namespace rp
+
+using namespace rp;
+
+// ---------------------------------------------------------------- C ABI
+extern "C" int rp_init(int device) {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return set_error(RP_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= count)
+    return set_error(RP_ERR_ARG, "device %d out of range (count %d)", device, count);
+  RP_HIP(hipSetDevice(device));
+  g_device = device;
+  return RP_OK;
+}
+
+extern "C" int rp_finalize(void) {
+  if (g_device >= 0) RP_HIP(hipDeviceSynchronize());
+  return RP_OK;
+}
+
+extern "C" int rp_last_error(char* buf, int len) {
+  if (!buf || len <= 0) return RP_ERR_ARG;
+  snprintf(buf, (size_t)len, "%s", g_err.c_str());
+  return RP_OK;
+}
+
+extern "C" int rp_version(void) { return 1; }
+
+extern "C" int rp_pes_create(int kind, int ndim, int natom, const double* dp, int ndp,
+                             const int* ip, int nip, rp_pes** out) {
+  RP_REQUIRE(out != nullptr, "rp_pes_create: out is NULL");
+  *out = nullptr;
+  RP_REQUIRE(ndim > 0 && natom > 0, "rp_pes_create: ndim=%d natom=%d", ndim, natom);
+  int rc = ensure_init();
+  if (rc) return rc;
+  const int D = ndim * natom;
+  int need = 0;
+  switch (kind) {
+    case PES_QUARTIC: need = 2; break;
+    case PES_COUPLED2D:
+      RP_REQUIRE(D == 2, "COUPLED2D needs ndof = 2 (got %d)", D);
+      need = 5;
+      break;
+    case PES_EVB:
+      need = 1 + D * D + 3 * D;
+      RP_REQUIRE(ip && nip == natom, "EVB needs the atom permutation (nip = natom)");
+      for (int a = 0; a < natom; ++a)
+        RP_REQUIRE(ip[a] >= 0 && ip[a] < natom, "EVB permutation entry %d out of range", a);
+      break;
+    case PES_WATER:
+      RP_REQUIRE(ndim == 3 && (natom == 3 || natom == 6 || natom == 18),
+                 "WATER needs ndim=3 and 1, 2 or 6 molecules (natom=%d)", natom);
+      need = 8;
+      break;
+    case PES_HARMONIC: need = 2 * D; break;
+    default: return set_error(RP_ERR_ARG, "unknown potential kind %d", kind);
+  }
+  RP_REQUIRE(dp && ndp >= need, "potential kind %d needs %d parameters (got %d)", kind, need, ndp);
+  rp_pes* P = new rp_pes();
+  P->kind = kind;
+  P->ndim = ndim;
+  P->natom = natom;
+  P->ndof = D;
+  P->v0 = 0.0;
+  P->dp.assign(dp, dp + need);
+  if (ip && nip > 0) P->ip.assign(ip, ip + nip);
+  std::vector<double> dev = P->dp;
+  if (kind == PES_EVB) {
+    // API {kappa, Q, lam, cub, xa} -> device {kappa, QA, xa, QB, xb, lam, cub}
+    // (Pi x)_j = x_{pi(j)}, pi(a*nd+d) = perm[a]*nd+d; QB(k,i) = Q(pi^-1(k), i), xb_k = xa_{pi^-1(k)}
+    const double* Q = dp + 1;
+    const double* lam = Q + D * D;
+    const double* cub = lam + D;
+    const double* xa = cub + D;
+    std::vector<int> pinv(D);
+    for (int a = 0; a < natom; ++a)
+      for (int d = 0; d < ndim; ++d) pinv[ip[a] * ndim + d] = a * ndim + d;
+    dev.assign(1 + 2 * D * D + 4 * D, 0.0);
+    dev[0] = dp[0];
+    double* QA = dev.data() + 1;
+    double* xA = QA + D * D;
+    double* QB = xA + D;
+    double* xB = QB + D * D;
+    double* L = xB + D;
+    double* Cc = L + D;
+    for (int k = 0; k < D * D; ++k) QA[k] = Q[k];
+    for (int k = 0; k < D; ++k) {
+      xA[k] = xa[k];
+      xB[k] = xa[pinv[k]];
+      L[k] = lam[k];
+      Cc[k] = cub[k];
+    }
+    for (int i = 0; i < D; ++i)
+      for (int k = 0; k < D; ++k) QB[k + D * i] = Q[pinv[k] + D * i];
+  }
+  RP_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+  if ((rc = P->d_dp.alloc(dev.size())) || (rc = P->d_dp.upload(dev.data(), dev.size(), P->stream))) {
+    delete P;
+    return rc;
+  }
+  std::vector<int> ipv = P->ip.empty() ? std::vector<int>{0} : P->ip;
+  if ((rc = P->d_ip.alloc(ipv.size())) || (rc = P->d_ip.upload(ipv.data(), ipv.size(), P->stream))) {
+    delete P;
+    return rc;
+  }
+  RP_HIP(hipStreamSynchronize(P->stream));
+  *out = P;
+  return RP_OK;
+}
+
+extern "C" int rp_pes_destroy(rp_pes* pes) {
+  if (!pes) return RP_OK;
+  (void)hipStreamSynchronize(pes->stream);
+  pes->d_dp.free();
+  pes->d_ip.free();
+  pes->d_x.free();
+  pes->d_g.free();
+  pes->d_e.free();
+  (void)hipStreamDestroy(pes->stream);
+  delete pes;
+  return RP_OK;
+}
+
+extern "C" int rp_pes_set_v0(rp_pes* pes, double v0) {
+  RP_REQUIRE(pes, "rp_pes_set_v0: NULL potential");
+  pes->v0 = v0;
+  return RP_OK;
+}
+
+static int pes_scratch(rp_pes* pes, size_t nx, size_t ng, size_t ne) {
+  int rc;
+  if (pes->d_x.n < nx && (rc = pes->d_x.alloc(nx))) return rc;
+  if (pes->d_g.n < ng && (rc = pes->d_g.alloc(ng))) return rc;
+  if (pes->d_e.n < ne && (rc = pes->d_e.alloc(ne))) return rc;
+  return RP_OK;
+}
+
+extern "C" int rp_pes_potforce(rp_pes* pes, int nbead, const double* x, double* grad,
+                               double* energy) {
+  RP_REQUIRE(pes && x && nbead >= 0, "rp_pes_potforce: bad arguments");
+  if (nbead == 0) return RP_OK;
+  int rc;
+  const size_t nx = (size_t)nbead * pes->ndof;
+  if ((rc = ensure_init()) || (rc = pes_scratch(pes, nx, nx, nbead))) return rc;
+  if ((rc = pes->d_x.upload(x, nx, pes->stream))) return rc;
+  if ((rc = launch_pes_eval(pes, nbead, nbead, pes->d_x.p, grad ? pes->d_g.p : nullptr,
+                            pes->d_e.p, pes->stream)))
+    return rc;
+  if (grad && (rc = pes->d_g.download(grad, nx, pes->stream))) return rc;
+  if (energy && (rc = pes->d_e.download(energy, nbead, pes->stream))) return rc;
+  RP_HIP(hipStreamSynchronize(pes->stream));
+  return RP_OK;
+}
+
+extern "C" int rp_pes_hessian(rp_pes* pes, int nbead, const double* x, double* hess) {
+  RP_REQUIRE(pes && x && hess && nbead >= 0, "rp_pes_hessian: bad arguments");
+  if (nbead == 0) return RP_OK;
+  RP_REQUIRE(pes->ndof <= 64, "rp_pes_hessian: ndof %d > 64", pes->ndof);
+  int rc;
+  const size_t nx = (size_t)nbead * pes->ndof, nh = nx * pes->ndof;
+  if ((rc = ensure_init()) || (rc = pes_scratch(pes, nx, nh, 1))) return rc;
+  if ((rc = pes->d_x.upload(x, nx, pes->stream))) return rc;
+  PesDev P = pes->dev();
+  hipStream_t s = pes->stream;
+  const double* dx = pes->d_x.p;
+  double* dh = pes->d_g.p;
+  rc = dispatch_pes(pes, [&]<int K, int M>() -> int {
+    pes_hess_kernel<K, M><<<nbead, 64, 0, s>>>(P, nbead, dx, dh);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  });
+  if (rc) return rc;
+  if ((rc = pes->d_g.download(hess, nh, s))) return rc;
+  RP_HIP(hipStreamSynchronize(s));
+  return RP_OK;
+}
+
+extern "C" int rp_pes_param_grad(rp_pes* pes, int nbead, const double* x, double* out,
+                                 int* nparam) {
+  RP_REQUIRE(pes && x && out && nbead >= 0, "rp_pes_param_grad: bad arguments");
+  if (pes->kind != PES_WATER)
+    return set_error(RP_ERR_ARG, "potential kind %d has no parameter sensitivities", pes->kind);
+  if (nparam) *nparam = 2;
+  if (nbead == 0) return RP_OK;
+  int rc;
+  const size_t nx = (size_t)nbead * pes->ndof;
+  if ((rc = ensure_init()) || (rc = pes_scratch(pes, nx, 2 * (size_t)nbead, 1))) return rc;
+  if ((rc = pes->d_x.upload(x, nx, pes->stream))) return rc;
+  PesDev P = pes->dev();
+  int nb = (nbead + 127) / 128;
+  switch (pes->natom) {
+    case 3: water_param_grad_kernel<1><<<nb, 128, 0, pes->stream>>>(P, nbead, pes->d_x.p, pes->d_g.p); break;
+    case 6: water_param_grad_kernel<2><<<nb, 128, 0, pes->stream>>>(P, nbead, pes->d_x.p, pes->d_g.p); break;
+    default: water_param_grad_kernel<6><<<nb, 128, 0, pes->stream>>>(P, nbead, pes->d_x.p, pes->d_g.p); break;
+  }
+  RP_HIP(hipGetLastError());
+  if ((rc = pes->d_g.download(out, 2 * (size_t)nbead, pes->stream))) return rc;
+  RP_HIP(hipStreamSynchronize(pes->stream));
+  return RP_OK;
+}
