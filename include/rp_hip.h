@@ -139,22 +139,34 @@ int rp_batch_create(rp_pes *pes, const double *mass, int npoly, const rp_polymer
                     double dhdrlimit, int cayley, uint64_t seed, rp_batch **out);
 int rp_batch_destroy(rp_batch *b);
 int rp_batch_set_path(rp_batch *b, int npath, const double *lampath, const double *path);
-/* init_path for every polymer from its Philox stream (or injected noise: npoly*n*ndof) */
+/* init_path for every polymer (verletmodule.f90:32-48, 102-115) from its Philox stream,
+ * or from injected draws: noise = each polymer's n*ndof draws in user order, one after
+ * the other */
 int rp_batch_init_path(rp_batch *b, const double *noise);
 int rp_batch_set_state(rp_batch *b, int poly, const double *x, const double *p);
 int rp_batch_get_state(rp_batch *b, int poly, double *x, double *p);
-/* advance every polymer nsteps steps; TI accumulates on steps after imin (counted from the
- * first rp_batch_run).  noise: NULL (Philox) or [step][poly][n*ndof] draws in the
- * reference's order, for parity runs.  Steps are issued in launches of `block` steps. */
-int rp_batch_run(rp_batch *b, int64_t nsteps, int64_t imin, const double *noise);
+/* Andersen parity runs: injected Poisson intervals, nkicks_per_poly per polymer
+ * (kicks[poly*nkicks_per_poly + k]); NULL returns to the Philox Poisson draws */
+int rp_batch_set_kicks(rp_batch *b, const int32_t *kicks, int64_t nkicks_per_poly);
+/* advance every polymer nsteps steps; TI accumulates on steps after imin (steps are
+ * counted from the first rp_batch_run).  noise: NULL (Philox) or injected draws for
+ * parity runs, nnoise_per_poly per polymer (noise[poly*nnoise_per_poly + k]) consumed in
+ * the reference's order by O-steps, resamples and re-initialisations. */
+int rp_batch_run(rp_batch *b, int64_t nsteps, int64_t imin, const double *noise,
+                 int64_t nnoise_per_poly);
 /* per polymer: running TI sum, accumulated step count, re-initialisations */
 int rp_batch_ti_readout(rp_batch *b, double *dhdr_sum, int64_t *count, int64_t *nreinit);
-/* launch shape: steps per launch (>= 1) and waves per polymer workgroup (4 or 8);
- * 0 keeps the current value.  rp_batch_autotune times the candidates on the live batch
- * (state is saved and restored) and keeps the fastest. */
+/* per polymer: Philox draw cursor (the stream position of its next normal) */
+int rp_batch_get_cursor(rp_batch *b, int64_t *cursor);
+/* families of equal (n, betan), one fused-kernel launch each: n[f], count[f] */
+int rp_batch_families(rp_batch *b, int *nfam, int *n, int *count);
+/* launch shape: steps per launch (>= 1; 0 keeps the current value) and waves per
+ * polymer workgroup (4; 0 keeps).  rp_batch_autotune times the step-block candidates on
+ * the live batch (state is saved and restored) and keeps the fastest. */
 int rp_batch_set_launch(rp_batch *b, int steps_per_launch, int waves);
 int rp_batch_autotune(rp_batch *b, int trial_steps, int *best_steps, int *best_waves);
-/* hipEvent-timed average duration of the last rp_batch_run's fused-step kernel launches */
+/* hipEvent-timed average duration of one fused-kernel launch per family in the last
+ * rp_batch_run: avg_ms[nfam]; launches = launches per family */
 int rp_batch_last_kernel_ms(rp_batch *b, double *avg_ms, int64_t *launches);
 
 /* ------------------------------------------------------------------ instantons

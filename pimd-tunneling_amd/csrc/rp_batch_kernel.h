@@ -1,0 +1,401 @@
+// rp_batch_kernel.h -- the fused persistent step kernel of the batched engine.
+// Included by the per-potential instantiation units (rp_batch_k_*.hip).
+#pragma once
+#include "rp_batch.h"
+
+namespace rp {
+
+constexpr int kBatchWaves = 4;
+
+__device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ double batch_draw(const BatchArgs& A, int64_t noff, uint64_t strm,
+                                             int64_t idx) {
+  if (A.noise) return A.noise[noff + idx];
+  return philox_normal(A.seed, strm, idx);
+}
+
+// acc = T S for all units of this wave (see rp_batch.h for the unit layout).
+template <int DT, int UPW>
+__device__ __forceinline__ void gemm_T(const double* __restrict__ T, int NPAD, const double* S,
+                                       int LD, int wave, int m, int g, d4 (&acc)[UPW][4]) {
+  constexpr int W = kBatchWaves;
+  const int units = (NPAD >> 5) * DT;
+#pragma unroll
+  for (int u = 0; u < UPW; ++u)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[u][c] = d4{0.0, 0.0, 0.0, 0.0};
+  const int nk = NPAD >> 2;
+  for (int kc = 0; kc < nk; ++kc) {
+    const int k = 4 * kc + g;
+    const double* Tk = T + (size_t)k * NPAD;
+    const double* Sk = S + k * LD;
+#pragma unroll
+    for (int u = 0; u < UPW; ++u) {
+      const int unit = wave + u * W;
+      if (unit < units) {
+        const int rp = unit / DT, dtile = unit - rp * DT;
+        const double a0 = Tk[rp * 32 + m];
+        const double a1 = Tk[rp * 32 + 16 + m];
+        const double bx = Sk[dtile * 16 + m];
+        const double bp = Sk[DT * 16 + dtile * 16 + m];
+        acc[u][0] = mfma_f64(a0, bx, acc[u][0]);
+        acc[u][1] = mfma_f64(a0, bp, acc[u][1]);
+        acc[u][2] = mfma_f64(a1, bx, acc[u][2]);
+        acc[u][3] = mfma_f64(a1, bp, acc[u][3]);
+      }
+    }
+  }
+}
+
+// S <- acc (x and/or p columns)
+template <int DT, int UPW>
+__device__ __forceinline__ void store_acc(double* S, int LD, int NPAD, int wave, int m, int g,
+                                          const d4 (&acc)[UPW][4], bool store_x) {
+  constexpr int W = kBatchWaves;
+  const int units = (NPAD >> 5) * DT;
+#pragma unroll
+  for (int u = 0; u < UPW; ++u) {
+    const int unit = wave + u * W;
+    if (unit < units) {
+      const int rp = unit / DT, dtile = unit - rp * DT;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int i = rp * 32 + h * 16 + g + 4 * j;
+          if (store_x) S[i * LD + dtile * 16 + m] = acc[u][2 * h][j];
+          S[i * LD + DT * 16 + dtile * 16 + m] = acc[u][2 * h + 1][j];
+        }
+    }
+  }
+}
+
+// Four normals for lane group g at modes i = ibase + g + 4j (j = 0..3) of one dof,
+// draw index base + i.  When N is even the draws of modes (i, i+1) share one Philox
+// block, so lane groups (2k, 2k+1) split the four blocks of their pairs and swap
+// halves with the partner lane 16 away: half the Philox/Box-Muller work.
+__device__ __forceinline__ void noise4(const BatchArgs& A, int64_t noff, uint64_t strm,
+                                       int64_t base, int ibase, int g, bool real_dof,
+                                       double (&z)[4]) {
+  const int N = A.N;
+  if (A.noise || (N & 1)) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = ibase + g + 4 * j;
+      z[j] = (real_dof && i < N) ? batch_draw(A, noff, strm, base + i) : 0.0;
+    }
+    return;
+  }
+  // base is even (the cursor and dof*N are multiples of N): pair (i even, i+1) = one block
+  const bool odd = g & 1;
+  const int i0 = ibase + (g & ~1);
+  double mine[2], other[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int ie = i0 + 4 * (odd ? 2 + t : t);  // even lane: j = 0,1; odd lane: j = 2,3
+    double z0 = 0.0, z1 = 0.0;
+    if (real_dof && ie < N) philox_normal2(A.seed, strm, (uint64_t)(base + ie) >> 1, z0, z1);
+    mine[t] = odd ? z1 : z0;
+    other[t] = odd ? z0 : z1;
+  }
+  const double r0 = __shfl_xor(other[0], 16), r1 = __shfl_xor(other[1], 16);
+  if (!odd) {
+    z[0] = mine[0]; z[1] = mine[1]; z[2] = r0; z[3] = r1;
+  } else {
+    z[0] = r0; z[1] = r1; z[2] = mine[0]; z[3] = mine[1];
+  }
+}
+
+// Normal-mode phase on the forward-transform accumulators: q = acc_x - beadvec,
+// A(dt/2) [O A(dt/2)] per element, then S <- [q + beadvec | p~].
+template <int DT, int UPW>
+__device__ __forceinline__ void nm_phase(const BatchArgs& A, double* S, const double* MT,
+                                         int wave, int m, int g, d4 (&acc)[UPW][4],
+                                         const double* __restrict__ pa, int64_t noff,
+                                         uint64_t strm, int64_t cur, bool with_o, bool& nan) {
+  constexpr int W = kBatchWaves;
+  const int N = A.N, D = A.D, LD = A.LD;
+  const int units = (A.NPAD >> 5) * DT;
+  const double c = A.c, sq = A.sq, sp = A.sp;
+#pragma unroll
+  for (int u = 0; u < UPW; ++u) {
+    const int unit = wave + u * W;
+    if (unit < units) {
+      const int rp = unit / DT, dtile = unit - rp * DT;
+      const int dof = dtile * 16 + m;
+      const bool dreal = dof < D;
+      const double mc = A.massd[dof], imc = 1.0 / mc, sqm = sqrt(mc);
+      const double av = pa[dof], bvv = pa[DT * 16 + dof];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        double z[4] = {0.0, 0.0, 0.0, 0.0};
+        if (with_o) noise4(A, noff, strm, cur + (int64_t)dof * N, rp * 32 + h * 16, g, dreal, z);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int i = rp * 32 + h * 16 + g + 4 * j;
+          const double* md = MT + i * kMODE;
+          const double wm = mc * md[MD_MU], iwm = imc * md[MD_IMU];
+          const double bv = av * md[MD_U] + bvv * md[MD_W];
+          double q = acc[u][2 * h][j] - bv;
+          double pt = acc[u][2 * h + 1][j];
+          double q1 = c * q + (sq * iwm) * pt;
+          double p1 = -(sp * wm) * q + c * pt;
+          nan |= (p1 != p1);
+          if (with_o) {
+            const double p2 = A.c1sq * p1 + (sqm * md[MD_LAM] * A.kappa) * z[j];
+            q = c * q1 + (sq * iwm) * p2;
+            pt = -(sp * wm) * q1 + c * p2;
+            nan |= (pt != pt);
+          } else {
+            q = q1;
+            pt = p1;
+          }
+          S[i * LD + dof] = q + bv;
+          S[i * LD + DT * 16 + dof] = pt;
+        }
+      }
+    }
+  }
+}
+
+// B: p -= dt grad V(x) for every bead (verletmodule.f90:565-585)
+template <int KIND, int MAXD, int DT>
+__device__ __forceinline__ void pes_kick(const BatchArgs& A, double* S, int tid, bool& nan) {
+  constexpr int W = kBatchWaves;
+  const int N = A.N, D = A.D, LD = A.LD;
+  for (int b = tid; b < N; b += W * 64) {
+    double xs[MAXD], gs[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) xs[d] = (d < D) ? S[b * LD + d] : 0.0;
+    (void)eval_kind<KIND, MAXD>(A.pes, xs, gs);
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+      if (d < D) {
+        double np = S[b * LD + DT * 16 + d] - gs[d] * A.dt;
+        nan |= (np != np);
+        S[b * LD + DT * 16 + d] = np;
+      }
+    }
+  }
+}
+
+// NM-space momenta of a resample / init_path (verletmodule.f90:102-109, 211-219):
+// v_i = sqrt(1/betan) z sqrt(beadmass_i), draws in (dim outer, atom inner) blocks of N.
+template <int DT>
+__device__ __forceinline__ void write_resample(const BatchArgs& A, double* S, const double* MT,
+                                               int tid, int64_t noff, uint64_t strm,
+                                               int64_t cur) {
+  constexpr int W = kBatchWaves, DC = DT * 16;
+  const int N = A.N, NPAD = A.NPAD, D = A.D, LD = A.LD, natom = A.natom, ndim = A.ndim;
+  for (int e = tid; e < NPAD * DC; e += W * 64) {
+    const int c = e / NPAD, i = e - c * NPAD;
+    double v = 0.0;
+    if (i < N && c < D) {
+      const int atom = c / ndim, dim = c - atom * ndim;
+      const double z = batch_draw(A, noff, strm, cur + (int64_t)(dim * natom + atom) * N + i);
+      v = (A.stdev0 * z) * (sqrt(A.massd[c]) * MT[i * kMODE + MD_LAM] * A.tau);
+    }
+    S[i * LD + DC + c] = v;
+  }
+}
+
+// Phases of one step.  Every phase is: pre-op, sync, acc = T S, sync, post-op, sync.
+//   Langevin (time_step_pile):  [kick] T [NM: A O A] T [store]
+//   Andersen (time_step_nm):    ([resample] T [store p]) T [NM: A] T [store] [kick]
+//                               T [NM: A] T [store]
+//   re-initialisation:          [resample] T [store p, path spline -> x]
+enum : int { PH_FWD_PILE = 0, PH_FWD_A = 1, PH_BWD = 2, PH_RESAMPLE = 3, PH_REINIT = 4 };
+
+template <int KIND, int MAXD, int DT, int UPW>
+__global__ void __launch_bounds__(kBatchWaves * 64)
+    batch_kernel(BatchArgs A, int thermostat) {
+  extern __shared__ double S[];
+  __shared__ int s_flag;
+  constexpr int W = kBatchWaves, DC = DT * 16;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, m = lane & 15, g = lane >> 4;
+  const int poly = A.first + blockIdx.x;
+  const int N = A.N, NPAD = A.NPAD, D = A.D, LD = A.LD, ndim = A.ndim, natom = A.natom;
+  double* MT = S + NPAD * LD;
+  int64_t* pi = A.pi64 + (size_t)poly * kPI64;
+  double* st = A.state + pi[PI_STATE_OFF];
+  const double* pa = A.pdata + (size_t)poly * 3 * DC;
+  const uint64_t strm = A.stream[poly];
+  const int64_t noff = pi[PI_NOISE_OFF];
+  const bool langevin = thermostat == RP_THERMO_LANGEVIN;
+  int64_t cur = pi[PI_CURSOR];
+  bool nan = false;
+  // state -> LDS (zero padding); per-mode table -> LDS
+  for (int e = tid; e < NPAD * 2 * DC; e += W * 64) {
+    const int c = e / NPAD, i = e - c * NPAD;
+    const int d = c < DC ? c : c - DC;
+    double v = 0.0;
+    if (i < N && d < D) v = st[(c < DC ? 0 : (size_t)D * N) + (size_t)d * N + i];
+    S[i * LD + c] = v;
+  }
+  for (int e = tid; e < NPAD * kMODE; e += W * 64) MT[e] = A.modes[e];
+  if (tid == 0) s_flag = 0;
+  double tisum = 0.0;
+  int64_t ticnt = 0, nre = 0;
+  int64_t acount = 0, rkick = 0, kcur = 0;
+  if (!langevin) {
+    acount = pi[PI_ACOUNT];
+    rkick = pi[PI_RKICK];
+    kcur = pi[PI_KCURSOR];
+  }
+  const int64_t koff = pi[PI_KICK_OFF];
+  const int64_t need = (int64_t)N * D;  // draws per O-step / resample / re-initialisation
+  __syncthreads();
+  bool stop = false;
+  for (int s = A.init_path ? -1 : 0; s < A.nsteps; ++s) {
+    const int64_t ii = A.step0 + s + 1;  // the reference's step counter
+    int ph[6], nph = 0;
+    bool resample = false;
+    if (s < 0) {
+      // init_path before the first step (pimd_par.f90:329)
+      if (A.noise && cur + need > A.noise_cap) { stop = true; break; }
+      ph[nph++] = PH_REINIT;
+    } else if (langevin) {
+      if (A.noise && cur + need > A.noise_cap) { stop = true; break; }
+      ph[nph++] = PH_FWD_PILE;
+      ph[nph++] = PH_BWD;
+    } else {
+      // propagate_pimd_nm (verletmodule.f90:202-234): Poisson-timed resample first
+      if (rkick < 0) {
+        if (A.kicks && kcur >= A.kick_cap) { stop = true; break; }
+        rkick = A.kicks ? A.kicks[koff + kcur]
+                        : philox_poisson(A.seed, strm, kcur, (double)A.noutput);
+        ++kcur;
+      }
+      acount = acount + 1;
+      if (acount >= rkick) {
+        if ((A.noise && cur + need > A.noise_cap) || (A.kicks && kcur >= A.kick_cap)) {
+          stop = true;
+          break;
+        }
+        resample = true;
+        ph[nph++] = PH_RESAMPLE;
+      }
+      ph[nph++] = PH_FWD_A;
+      ph[nph++] = PH_BWD;
+      ph[nph++] = PH_FWD_A;
+      ph[nph++] = PH_BWD;
+    }
+    int nbwd = 0;
+    bool ti_done = false;
+#pragma unroll 1
+    for (int k = 0; k < nph; ++k) {
+      const int p = ph[k];
+      if (p == PH_FWD_PILE || (p == PH_FWD_A && nbwd == 1)) {
+        pes_kick<KIND, MAXD, DT>(A, S, tid, nan);  // B(dt)
+        __syncthreads();
+      } else if (p == PH_RESAMPLE || p == PH_REINIT) {
+        write_resample<DT>(A, S, MT, tid, noff, strm, cur);
+        __syncthreads();
+      }
+      d4 acc[UPW][4];
+      gemm_T<DT, UPW>(A.T, NPAD, S, LD, wave, m, g, acc);
+      __syncthreads();
+      if (p == PH_FWD_PILE || p == PH_FWD_A) {
+        nm_phase<DT, UPW>(A, S, MT, wave, m, g, acc, pa, noff, strm, cur, p == PH_FWD_PILE, nan);
+        if (p == PH_FWD_PILE) cur += need;
+      } else {
+        store_acc<DT, UPW>(S, LD, NPAD, wave, m, g, acc, p == PH_BWD);
+        if (p == PH_BWD) ++nbwd;
+        if (p == PH_REINIT) {
+          // init_path positions: x_k = splint(path, (k-1) xi/(n-1)) (verletmodule.f90:40-48)
+          const double xi = A.pf64[(size_t)poly * kPF64 + PF_XI];
+          const double* lampath = A.pathd;
+          for (int e = tid; e < N * D; e += W * 64) {
+            const int cc = e / N, i = e - cc * N;
+            const double xe = (double)i * xi / (double)(N - 1);
+            S[i * LD + cc] = splint_dev(lampath, lampath + A.npath + (size_t)cc * A.npath,
+                                        lampath + A.npath * (1 + D) + (size_t)cc * A.npath,
+                                        A.npath, xe);
+          }
+        }
+        if (p == PH_RESAMPLE || p == PH_REINIT) cur += need;
+      }
+      __syncthreads();
+      // TI estimator after the step (verletmodule.f90:397-410, 236-244)
+      if (k == nph - 1 && !ti_done && s >= 0 && ii > A.imin) {
+        ti_done = true;
+        if (tid == 0) {
+          double contr = 0.0;
+          for (int jd = 0; jd < ndim; ++jd)
+            for (int ka = 0; ka < natom; ++ka) {
+              const int d = ka * ndim + jd;
+              contr = contr + A.massd[d] * (-S[(N - 1) * LD + d]) * pa[2 * DC + d];
+            }
+          const bool ok = !langevin || fabs(contr) < A.limit || A.limit < 0.0;
+          if (ok) {
+            tisum = tisum + contr;
+            ++ticnt;
+          }
+          s_flag = ok ? 0 : 1;
+        }
+        if (langevin && A.limit >= 0.0) {
+          __syncthreads();
+          if (s_flag) {
+            if (A.noise && cur + need > A.noise_cap) {
+              stop = true;
+            } else {
+              ph[nph++] = PH_REINIT;  // one more phase: re-initialise the path
+              ++nre;
+            }
+          }
+        }
+      }
+    }
+    if (stop) break;
+    if (resample) {
+      rkick = A.kicks ? A.kicks[koff + kcur] : philox_poisson(A.seed, strm, kcur, (double)A.noutput);
+      ++kcur;
+      acount = 0;
+    }
+  }
+  if (stop && tid == 0) atomicOr(&A.flags[1], 1);
+  __syncthreads();
+  // LDS -> state
+  for (int e = tid; e < N * 2 * D; e += W * 64) {
+    const int c = e / N, i = e - c * N;
+    const int d = c < D ? c : c - D;
+    const int sc = c < D ? d : DC + d;
+    st[(c < D ? 0 : (size_t)D * N) + (size_t)d * N + i] = S[i * LD + sc];
+  }
+  if (nan) atomicOr(&A.flags[0], 1);
+  if (tid == 0) {
+    A.pf64[(size_t)poly * kPF64 + PF_TISUM] += tisum;
+    pi[PI_TICNT] += ticnt;
+    pi[PI_NREINIT] += nre;
+    pi[PI_CURSOR] = cur;
+    if (!langevin) {
+      pi[PI_ACOUNT] = acount;
+      pi[PI_RKICK] = rkick;
+      pi[PI_KCURSOR] = kcur;
+    }
+  }
+}
+
+template <int KIND, int MAXD, int DT>
+int launch_batch_kind(const BatchArgs& a, int thermostat, int npoly, hipStream_t s) {
+  constexpr int W = kBatchWaves;
+  const int units = (a.NPAD >> 5) * DT;
+  const int upw = (units + W - 1) / W;
+  const size_t lds = batch_lds_bytes(a.NPAD, DT);
+  auto go = [&](auto kern) -> int {
+    RP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+    kern<<<npoly, W * 64, lds, s>>>(a, thermostat);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  };
+  if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1>);
+  if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2>);
+  if (upw <= 4) return go(batch_kernel<KIND, MAXD, DT, 4>);
+  return set_error(RP_ERR_ARG, "batch: %d MFMA units per wave is more than supported", upw);
+}
+
+}  // namespace rp

@@ -9,5 +9,6 @@ from . import specs
 from ._lib import RpError, init, lib
 from .pes import Pes
 from .verlet import Chain
+from .batch import Batch
 
-__all__ = ["specs", "RpError", "init", "lib", "Pes", "Chain"]
+__all__ = ["specs", "RpError", "init", "lib", "Pes", "Chain", "Batch"]

@@ -76,7 +76,10 @@ _SIGS = {
     "rp_batch_init_path": [vp, dp],
     "rp_batch_set_state": [vp, C.c_int, dp, dp],
     "rp_batch_get_state": [vp, C.c_int, dp, dp],
-    "rp_batch_run": [vp, C.c_int64, C.c_int64, dp],
+    "rp_batch_run": [vp, C.c_int64, C.c_int64, dp, C.c_int64],
+    "rp_batch_set_kicks": [vp, i32p, C.c_int64],
+    "rp_batch_get_cursor": [vp, i64p],
+    "rp_batch_families": [vp, ip, ip, ip],
     "rp_batch_ti_readout": [vp, dp, i64p, i64p],
     "rp_batch_set_launch": [vp, C.c_int, C.c_int],
     "rp_batch_autotune": [vp, C.c_int, ip, ip],

@@ -1,0 +1,179 @@
+"""Parity of the batched engine (fused persistent step kernel) against the oracle.
+
+Every polymer of a batch must follow propagate_pimd_pile / propagate_pimd_nm
+(verletmodule.f90:190-250, 372-416) exactly: with injected draws, and with the
+engine's own Philox streams (which the oracle replays).  Trajectories of 10-100 steps
+are compared at 1e-10 normwise relative; TI sums at 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import oracle_pes, rand_chain, rel
+from rptunnel import Batch, Chain, Pes, specs
+
+pytestmark = pytest.mark.gpu
+
+TAU, DT, GAMMA, BETA = 20.0, 1.0, 1e-3, 2000.0
+
+
+def _polymers(sp, ns, xis=None):
+    pols = []
+    for k, n in enumerate(ns):
+        xi = 0.5 if xis is None else xis[k]
+        b = sp.well1 + xi * (sp.well2 - sp.well1)
+        pols.append(dict(n=n, betan=BETA / (n + 1), xi=xi, a=sp.well1, b=b,
+                         dbdl=sp.well2 - sp.well1, stream=100 + k))
+    return pols
+
+
+def _state(sp, n, seed):
+    rng = np.random.default_rng(seed)
+    x = rand_chain(sp, n, seed, scale=0.02)
+    p = 0.3 * rng.standard_normal(x.shape) * np.sqrt(sp.mass.repeat(sp.ndim))[:, None] * 0.01
+    return x, p
+
+
+def _oracle_nm(sp, pol, cayley=False, gamma=GAMMA):
+    return O.NM(pol["n"], sp.ndim, sp.natom, sp.mass, pol["betan"], TAU, DT, gamma, cayley,
+                pol["a"], pol["b"])
+
+
+@pytest.mark.parametrize("name,ns", [("evb", [32, 64, 96]), ("evb", [128, 40]),
+                                     ("quartic", [64, 17]), ("coupled2d", [128, 32]),
+                                     ("water6", [32]), ("water2", [64])])
+def test_batch_pile_injected(name, ns):
+    sp = {"evb": specs.evb, "quartic": specs.quartic, "coupled2d": specs.coupled2d,
+          "water6": lambda: specs.water_cluster(nmol=6),
+          "water2": lambda: specs.water_cluster(nmol=2)}[name]()
+    pes = Pes(sp)
+    op = oracle_pes(sp)
+    pols = _polymers(sp, ns)
+    steps, imin = 24, 5
+    b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, seed=9)
+    cap = steps * max(ns) * sp.ndof
+    noise = np.random.default_rng(1).standard_normal((len(pols), cap))
+    states = []
+    for k, pol in enumerate(pols):
+        x, p = _state(sp, pol["n"], 10 + k)
+        b.set_state(k, x, p)
+        states.append((x, p))
+    b.run(steps, imin=imin, noise=noise)
+    sums, cnt, _ = b.ti_readout()
+    for k, pol in enumerate(pols):
+        nm = _oracle_nm(sp, pol)
+        x, p = states[k]
+        xo, po, dho, _ = O.propagate_pile(nm, op, O.Rng(normals=noise[k]), x, p, steps,
+                                          pol["dbdl"], imin=imin)
+        xg, pg = b.get_state(k)
+        assert rel(xg, xo) <= 1e-10, (k, rel(xg, xo))
+        assert rel(pg, po) <= 1e-10, (k, rel(pg, po))
+        assert cnt[k] == steps - imin
+        assert abs(sums[k] / (steps - imin) - dho) <= 1e-10 * abs(dho)
+
+
+def test_batch_pile_philox_and_blocks():
+    """Philox streams, several launches (steps per launch < steps), Cayley off/on."""
+    sp = specs.evb()
+    pes = Pes(sp)
+    op = oracle_pes(sp)
+    for cayley in (False, True):
+        pols = _polymers(sp, [64, 32, 64])
+        b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, cayley=cayley, seed=77)
+        b.set_launch(steps_per_launch=7)
+        st = []
+        for k, pol in enumerate(pols):
+            x, p = _state(sp, pol["n"], 20 + k)
+            b.set_state(k, x, p)
+            st.append((x, p))
+        b.run(30)
+        cur = b.cursors()
+        for k, pol in enumerate(pols):
+            nm = _oracle_nm(sp, pol, cayley=cayley)
+            r = O.Rng(seed=77, stream=pol["stream"])
+            xo, po, dho, _ = O.propagate_pile(nm, op, r, st[k][0], st[k][1], 30, pol["dbdl"])
+            xg, pg = b.get_state(k)
+            assert cur[k] == r.cursor
+            assert rel(xg, xo) <= 1e-10 and rel(pg, po) <= 1e-10
+
+
+def test_batch_init_path_and_reinit():
+    sp = specs.evb()
+    pes = Pes(sp)
+    op = oracle_pes(sp)
+    lampath, path = specs.linear_path(sp.well1, sp.well2, npath=7)
+    path = path + 0.02 * np.sin(np.arange(7))[None, :]
+    opath = O.Path(lampath, path)
+    pols = _polymers(sp, [32, 64], xis=[0.6, 0.3])
+    b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, dhdrlimit=300.0, seed=5)
+    b.set_path(lampath, path)
+    b.init_path()
+    # init_path from the Philox streams
+    inits = []
+    for k, pol in enumerate(pols):
+        nm = _oracle_nm(sp, pol)
+        r = O.Rng(seed=5, stream=pol["stream"])
+        xo, po = O.init_path(nm, opath, r, pol["xi"])
+        xg, pg = b.get_state(k)
+        assert rel(xg, xo) <= 1e-12 and rel(pg, po) <= 1e-12
+        inits.append((xo, po, r.cursor))
+    # dHdrlimit re-initialisations mid-run
+    b.run(60)
+    sums, cnt, nre = b.ti_readout()
+    for k, pol in enumerate(pols):
+        nm = _oracle_nm(sp, pol)
+        r = O.Rng(seed=5, stream=pol["stream"])
+        r.s.cursor = inits[k][2]
+        xo, po, dho, nreo = O.propagate_pile(nm, op, r, inits[k][0], inits[k][1], 60, pol["dbdl"],
+                                             dhdrlimit=300.0, xi=pol["xi"], path=opath)
+        assert nre[k] == nreo
+        xg, pg = b.get_state(k)
+        assert rel(xg, xo) <= 1e-10
+        assert abs(sums[k] / 60 - dho) <= 1e-10 * max(abs(dho), 1e-300)
+
+
+def test_batch_andersen_injected():
+    sp = specs.quartic()
+    pes = Pes(sp)
+    op = oracle_pes(sp)
+    pols = _polymers(sp, [64, 32])
+    b = Batch(pes, sp.mass, pols, TAU, DT, thermostat=1, noutput=5, seed=3)
+    kicks = np.array([[3, 5, 2, 7, 4, 6, 3, 5, 2, 8, 4, 4, 9, 3, 5],
+                      [2, 2, 3, 9, 1, 4, 6, 2, 5, 3, 3, 7, 2, 2, 4]], dtype=np.int32)
+    b.set_kicks(kicks)
+    cap = kicks.shape[1] * 64 * sp.ndof
+    noise = np.random.default_rng(4).standard_normal((2, cap))
+    st = []
+    for k, pol in enumerate(pols):
+        x, p = _state(sp, pol["n"], 30 + k)
+        b.set_state(k, x, p)
+        st.append((x, p))
+    b.run(40, imin=4, noise=noise)
+    sums, cnt, _ = b.ti_readout()
+    for k, pol in enumerate(pols):
+        nm = _oracle_nm(sp, pol)
+        xo, po, dho = O.propagate_andersen(nm, op, O.Rng(normals=noise[k], kicks=kicks[k]),
+                                           st[k][0], st[k][1], 40, pol["dbdl"], 5, imin=4)
+        xg, pg = b.get_state(k)
+        assert rel(xg, xo) <= 1e-10 and rel(pg, po) <= 1e-10
+        assert abs(sums[k] / 36 - dho) <= 1e-10 * abs(dho)
+
+
+def test_batch_equals_plain_path():
+    """The drop-in plain path and the batched engine run the same trajectory."""
+    sp = specs.evb()
+    pes = Pes(sp)
+    pol = _polymers(sp, [64])[0]
+    x, p = _state(sp, 64, 40)
+    b = Batch(pes, sp.mass, [pol], TAU, DT, GAMMA, seed=123)
+    b.set_state(0, x, p)
+    b.run(50, imin=10)
+    sums, _, _ = b.ti_readout()
+    ch = Chain(pes, 64, sp.mass, pol["betan"], TAU, DT, GAMMA, pol["a"], pol["b"])
+    ch.set_state(x, p)
+    ch.set_rng(123, pol["stream"])
+    dh, _ = ch.propagate_pimd_pile(50, pol["dbdl"], imin=10)
+    xg, _ = b.get_state(0)
+    xc, _ = ch.get_state()
+    assert rel(xg, xc) <= 1e-10
+    assert abs(sums[0] / 40 - dh) <= 1e-10 * abs(dh)
