@@ -1,0 +1,233 @@
+"""Headline benchmark: bead-steps/s of PIMD-TI on one MI355X, batched vs plain GPU path.
+
+Workload (the 27-DOF EVB double-well batched-TI configuration): 9 atoms, 27 dof,
+N in {32, 64, 96, 128, 192, 256} beads, 512 trajectories x 11 Gauss-Legendre lambda
+nodes at every N (33,792 polymers of uneven size), beta = 2000, tau = 20, dt = 1,
+gamma = 1e-3, Langevin (PILE) thermostat, FP64 throughout, synthetic seeded inputs.
+A "step" is one MD step of every polymer (B A O A + TI accumulation).
+
+Reported beside the batched value:
+  * the plain per-trajectory GPU path (the drop-in, one trajectory per call sequence)
+    timed on a bounded sample of the same trajectories (every N);
+  * the roofline of the dominant fused-kernel launch (FP64 MFMA, 78.6 TFLOP/s), from
+    hipEvent timing of the launches inside the timed region;
+  * the CPU baseline: the oracle's restatement of the reference algorithm (dense
+    transforms, reference step order) on the host cores, bounded sample at every N.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pimd-tunneling_amd", "python"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "bead-steps/s, batched vs plain GPU, at equal TI splitting error (1 GPU)"
+NS = [32, 64, 96, 128, 192, 256]
+NTRAJ, NLAMBDA = 512, 11
+BETA, TAU, DT, GAMMA = 2000.0, 20.0, 1.0, 1e-3
+PEAK_FP64 = 78.6e12
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def flops_per_bead_step(n, D):
+    """Algorithmic FP64 work of one bead-step: two dense N x N transforms of the 2D
+    state columns (8 N D), the elementwise kick / rotations / O-step / shifts (21 D), and
+    the EVB potential: two wells, each y = W^T (x - xw) and W h (2 x 2 D^2) plus ~13 D."""
+    w_pes = 8 * D * D + 26 * D
+    return 8 * n * D + 21 * D + w_pes
+
+
+def build_polymers(sp, ns, ntraj, nlam):
+    from rptunnel import specs
+    import oracle as O  # noqa: F401  (not used: node placement is driver arithmetic)
+
+    xi, _ = gauleg(nlam)
+    lampath, path = specs.linear_path(sp.well1, sp.well2, npath=2)
+    pols = []
+    stream = 0
+    for n in ns:
+        for il, x in enumerate(xi):
+            b = sp.well1 + x * (sp.well2 - sp.well1)
+            for r in range(ntraj):
+                pols.append(dict(n=n, betan=BETA / (n + 1), xi=float(x), a=sp.well1, b=b,
+                                 dbdl=sp.well2 - sp.well1, stream=stream))
+                stream += 1
+    return pols, lampath, path
+
+
+def gauleg(n):
+    from rptunnel import ti
+    return ti.gauleg(0.0, 1.0, n)
+
+
+def plain_sample(pes, sp, ns, steps):
+    """Seconds per polymer-step of the plain path at each N (one trajectory, host loop
+    over per-step launch sequences, as the reference driver steps one trajectory)."""
+    from rptunnel import Chain
+
+    out = {}
+    dbdl = sp.well2 - sp.well1
+    for n in ns:
+        betan = BETA / (n + 1)
+        b = sp.well1 + 0.5 * (sp.well2 - sp.well1)
+        ch = Chain(pes, n, sp.mass, betan, TAU, DT, GAMMA, sp.well1, b)
+        t = (np.arange(n) + 1.0) / (n + 1.0)
+        x = np.outer(sp.well1, 1 - t) + np.outer(b, t)
+        ch.set_state(x, np.zeros_like(x))
+        ch.set_rng(7, n)
+        ch.propagate_pimd_pile(5, dbdl)  # warm-up
+        t0 = time.perf_counter()
+        ch.propagate_pimd_pile(steps, dbdl)
+        out[n] = (time.perf_counter() - t0) / steps
+        ch.close()
+    return out
+
+
+def cpu_sample(sp, ns, budget_s, threads):
+    """Oracle restatement on the host cores: seconds per polymer-step at each N."""
+    import oracle as O
+
+    op = O.Pes(sp.kind, sp.ndim, sp.natom, sp.dp, sp.ip)
+    out, detail = {}, {}
+    for n in ns:
+        betan = BETA / (n + 1)
+        b = sp.well1 + 0.5 * (sp.well2 - sp.well1)
+        t = (np.arange(n) + 1.0) / (n + 1.0)
+        x0 = np.outer(sp.well1, 1 - t) + np.outer(b, t)
+        # calibrate: one trajectory, few steps
+        t1, _ = O.bench_pile(op, n, sp.ndim, sp.natom, sp.mass, betan, TAU, DT, GAMMA, sp.well1,
+                             b, sp.well2 - sp.well1, x0, 1, 4, 1, 1)
+        per = t1 / 4
+        steps = max(4, int(budget_s / len(ns) / per))
+        wall, _ = O.bench_pile(op, n, sp.ndim, sp.natom, sp.mass, betan, TAU, DT, GAMMA,
+                               sp.well1, b, sp.well2 - sp.well1, x0, threads, steps, threads, 1)
+        out[n] = wall / (threads * steps)  # seconds per polymer-step, all cores busy
+        detail[n] = {"traj": threads, "steps": steps, "wall_s": round(wall, 3)}
+    return out, detail
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--ntraj", type=int, default=NTRAJ)
+    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): every trajectory "
+              "is independent, so several GPUs would each run this same batch; see DESIGN.md "
+              "for how the path shards.", file=sys.stderr)
+        sys.exit(2)
+
+    import rptunnel
+    from rptunnel import Batch, Pes, specs
+
+    sp = specs.evb()
+    D = sp.ndof
+    pes = Pes(sp)
+    pols, lampath, path = build_polymers(sp, NS, args.ntraj, NLAMBDA)
+    log(f"creating batch of {len(pols)} polymers (N = {NS}, {args.ntraj} traj x {NLAMBDA} lambda)")
+    batch = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, seed=2024)
+    batch.set_path(lampath, path)
+    batch.init_path()
+    fams = batch.families()
+    if not args.no_autotune:
+        best_steps, _ = batch.autotune(trial_steps=16)
+        log(f"autotune: {best_steps} steps per launch")
+    log(f"warm-up {args.warmup} steps")
+    batch.run(args.warmup)
+    import ctypes
+    rptunnel.lib().rp_finalize()  # device synchronize
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    batch.run(args.steps)
+    rptunnel.lib().rp_finalize()
+    wall = time.perf_counter() - t0
+    beads_per_step = sum(p["n"] for p in pols)
+    value = beads_per_step * args.steps / wall
+    fam_ms, launches = batch.last_kernel_ms()
+    sums, cnt, _ = batch.ti_readout()
+    assert np.all(np.isfinite(sums)), "non-finite TI sums"
+    log(f"batched: {value:.4e} bead-steps/s ({1e3 * wall / args.steps:.3f} ms/step)")
+
+    # roofline of the dominant launch (the family with the largest summed launch time)
+    steps_per_launch = int(np.ceil(args.steps / launches)) if launches else args.steps
+    per_launch_steps = args.steps / launches
+    fam_rows = []
+    for (n, count), ms in zip(fams, fam_ms):
+        fl = per_launch_steps * count * n * flops_per_bead_step(n, D)
+        fam_rows.append({"n": n, "polymers": count, "avg_launch_ms": round(ms, 4),
+                         "tflops": round(fl / (ms * 1e-3) / 1e12, 3)})
+    dom = max(range(len(fam_rows)), key=lambda i: fam_ms[i])
+    dn, dcount = fams[dom]
+    dflops = per_launch_steps * dcount * dn * flops_per_bead_step(dn, D)
+    achieved = dflops / (fam_ms[dom] * 1e-3)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "batch_evb27_n256_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    log(f"plain path sample ({args.plain_steps} steps per N)")
+    tp = plain_sample(pes, sp, NS, args.plain_steps)
+    polys_per_n = args.ntraj * NLAMBDA
+    plain_value = sum(n * polys_per_n for n in NS) / sum(tp[n] * polys_per_n for n in NS)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: oracle on {threads} threads, ~{args.cpu_seconds:.0f} s")
+        tc, detail = cpu_sample(sp, NS, args.cpu_seconds, threads)
+        cpu_value = sum(n for n in NS) / sum(tc[n] for n in NS)
+        cpu = {"value": cpu_value, "unit": "bead-steps/s", "cores": threads, "kind": "port",
+               "sample": "oracle restatement of the reference PILE step (10 dense transforms "
+                         "per step, reference step order), one trajectory per thread, "
+                         f"at every N of the mix: {detail}; equal polymer counts per N"}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "bead-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "evb27-batched-ti", "natom": 9, "ndof": D, "N": NS,
+                   "traj_per_N": args.ntraj, "lambda_nodes": NLAMBDA, "polymers": len(pols),
+                   "beta": BETA, "tau": TAU, "dt": DT, "gamma": GAMMA,
+                   "thermostat": "langevin", "steps_per_launch": steps_per_launch},
+        "plain_gpu": {"value": plain_value, "unit": "bead-steps/s",
+                      "sec_per_polymer_step": {str(n): tp[n] for n in NS},
+                      "sample": f"one trajectory per N, {args.plain_steps} steps, same step"},
+        "speedup_vs_plain": value / plain_value,
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": traffic,
+                     "kernel": f"batch_kernel N={dn} family ({dcount} polymers)",
+                     "flops_per_bead_step": flops_per_bead_step(dn, D),
+                     "families": fam_rows},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

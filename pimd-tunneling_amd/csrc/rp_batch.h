@@ -77,8 +77,18 @@ inline int batch_dtiles(const rp_pes* pes) {
   return 2;
 }
 
-inline size_t batch_lds_bytes(int npad, int dt_tiles) {
-  return ((size_t)npad * (2 * dt_tiles * 16 + 1) + (size_t)npad * kMODE) * sizeof(double);
+// Doubles of potential parameters staged in LDS by the fused kernel (the EVB matrices;
+// other potentials read their few parameters from global memory).
+inline int batch_pes_lds(const rp_pes* pes, int N) {
+  (void)N;
+  if (pes->kind != PES_EVB) return 0;
+  const int M = evb_stride(pes->ndof);
+  return 1 + 2 * (M * M + M) + 2 * M;
+}
+
+inline size_t batch_lds_bytes(int npad, int dt_tiles, int pes_lds) {
+  return ((size_t)npad * (2 * dt_tiles * 16 + 1) + (size_t)npad * kMODE + (size_t)pes_lds) *
+         sizeof(double);
 }
 
 }  // namespace rp

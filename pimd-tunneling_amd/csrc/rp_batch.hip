@@ -187,7 +187,7 @@ extern "C" int rp_batch_create(rp_pes* pes, const double* mass, int npoly,
     if (!(desc[p].betan > 0) || !desc[p].a || !desc[p].b || !desc[p].dbdl)
       return fail(set_error(RP_ERR_ARG, "polymer %d: bad descriptor", p));
     const int npad = (n + 31) / 32 * 32;
-    if (batch_lds_bytes(npad, b->DT) > 160 * 1024)
+    if (batch_lds_bytes(npad, b->DT, batch_pes_lds(pes, n)) > 160 * 1024)
       return fail(set_error(RP_ERR_ARG, "polymer %d: n=%d with %d dof needs more LDS than a CU has",
                             p, n, b->D));
   }

@@ -28,24 +28,49 @@ __device__ __forceinline__ void gemm_T(const double* __restrict__ T, int NPAD, c
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[u][c] = d4{0.0, 0.0, 0.0, 0.0};
   const int nk = NPAD >> 2;
-  for (int kc = 0; kc < nk; ++kc) {
+  // Software pipeline: the fragments of K-chunk kc+1 are loaded while the MFMAs of
+  // chunk kc run (one wavefront per SIMD has no other wave to hide load latency).
+  // wave is wave-uniform (an SGPR), so the unit loop has no divergent control flow; a
+  // slot past the last unit recomputes the last unit and its result is dropped by the
+  // consumers, which keeps the accumulators in AGPRs through the loop.
+  int tcol[UPW], scol[UPW];
+#pragma unroll
+  for (int u = 0; u < UPW; ++u) {
+    const int unit = min(wave + u * W, units - 1);
+    const int rp = unit / DT, dtile = unit - rp * DT;
+    tcol[u] = rp * 32 + m;
+    scol[u] = dtile * 16 + m;
+  }
+  double fa0[UPW], fa1[UPW], fbx[UPW], fbp[UPW];
+  auto load = [&](int kc) {
     const int k = 4 * kc + g;
     const double* Tk = T + (size_t)k * NPAD;
     const double* Sk = S + k * LD;
 #pragma unroll
     for (int u = 0; u < UPW; ++u) {
-      const int unit = wave + u * W;
-      if (unit < units) {
-        const int rp = unit / DT, dtile = unit - rp * DT;
-        const double a0 = Tk[rp * 32 + m];
-        const double a1 = Tk[rp * 32 + 16 + m];
-        const double bx = Sk[dtile * 16 + m];
-        const double bp = Sk[DT * 16 + dtile * 16 + m];
-        acc[u][0] = mfma_f64(a0, bx, acc[u][0]);
-        acc[u][1] = mfma_f64(a0, bp, acc[u][1]);
-        acc[u][2] = mfma_f64(a1, bx, acc[u][2]);
-        acc[u][3] = mfma_f64(a1, bp, acc[u][3]);
-      }
+      fa0[u] = Tk[tcol[u]];
+      fa1[u] = Tk[tcol[u] + 16];
+      fbx[u] = Sk[scol[u]];
+      fbp[u] = Sk[DT * 16 + scol[u]];
+    }
+  };
+  load(0);
+  for (int kc = 0; kc < nk; ++kc) {
+    double a0[UPW], a1[UPW], bx[UPW], bp[UPW];
+#pragma unroll
+    for (int u = 0; u < UPW; ++u) {
+      a0[u] = fa0[u];
+      a1[u] = fa1[u];
+      bx[u] = fbx[u];
+      bp[u] = fbp[u];
+    }
+    if (kc + 1 < nk) load(kc + 1);
+#pragma unroll
+    for (int u = 0; u < UPW; ++u) {
+      acc[u][0] = mfma_f64(a0[u], bx[u], acc[u][0]);
+      acc[u][1] = mfma_f64(a0[u], bp[u], acc[u][1]);
+      acc[u][2] = mfma_f64(a1[u], bx[u], acc[u][2]);
+      acc[u][3] = mfma_f64(a1[u], bp[u], acc[u][3]);
     }
   }
 }
@@ -163,14 +188,18 @@ __device__ __forceinline__ void nm_phase(const BatchArgs& A, double* S, const do
 
 // B: p -= dt grad V(x) for every bead (verletmodule.f90:565-585)
 template <int KIND, int MAXD, int DT>
-__device__ __forceinline__ void pes_kick(const BatchArgs& A, double* S, int tid, bool& nan) {
+__device__ __forceinline__ void pes_kick(const BatchArgs& A, double* S, const double* PL, int tid,
+                                         bool& nan) {
   constexpr int W = kBatchWaves;
   const int N = A.N, D = A.D, LD = A.LD;
   for (int b = tid; b < N; b += W * 64) {
     double xs[MAXD], gs[MAXD];
 #pragma unroll
     for (int d = 0; d < MAXD; ++d) xs[d] = (d < D) ? S[b * LD + d] : 0.0;
-    (void)eval_kind<KIND, MAXD>(A.pes, xs, gs);
+    if constexpr (KIND == PES_EVB)
+      (void)evb_eval_at<MAXD>(PL, xs, gs);  // matrices staged in LDS
+    else
+      (void)eval_kind<KIND, MAXD>(A.pes, xs, gs);
 #pragma unroll
     for (int d = 0; d < MAXD; ++d) {
       if (d < D) {
@@ -178,6 +207,104 @@ __device__ __forceinline__ void pes_kick(const BatchArgs& A, double* S, int tid,
         nan |= (np != np);
         S[b * LD + DT * 16 + d] = np;
       }
+    }
+  }
+}
+
+// Threads per bead for the EVB kick: every thread of the workgroup works when N < 256.
+__host__ __device__ inline int evb_kick_tpb(int N) {
+  return N <= 32 ? 8 : N <= 64 ? 4 : N <= 128 ? 2 : 1;
+}
+
+// Sum over the TPB adjacent lanes of a group (TPB = 2, 4 or 8) with DPP lane moves:
+// quad_perm xor 1, quad_perm xor 2, then row_half_mirror pairs the two quads of 8.
+__device__ __forceinline__ double dpp_f64(double v, int ctrl) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  int nlo, nhi;
+  switch (ctrl) {
+    case 0xB1:
+      nlo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, false);
+      nhi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, false);
+      break;
+    case 0x4E:
+      nlo = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, false);
+      nhi = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, false);
+      break;
+    default:
+      nlo = __builtin_amdgcn_mov_dpp(lo, 0x141, 0xF, 0xF, false);
+      nhi = __builtin_amdgcn_mov_dpp(hi, 0x141, 0xF, 0xF, false);
+      break;
+  }
+  return __hiloint2double(nhi, nlo);
+}
+
+template <int TPB>
+__device__ __forceinline__ double group_sum(double v) {
+  if constexpr (TPB >= 2) v += dpp_f64(v, 0xB1);
+  if constexpr (TPB >= 4) v += dpp_f64(v, 0x4E);
+  if constexpr (TPB >= 8) v += dpp_f64(v, 0x141);
+  return v;
+}
+
+// B for the EVB with TPB adjacent lanes per bead (bead b = tid / TPB, part r = tid % TPB).
+// Part r holds the coordinates j = r, r+TPB, ...: each y_i = W_i . x - c_i is a partial dot
+// product summed over the group, and each part accumulates the gradient of its own j.
+template <int M, int DT, int TPB>
+__device__ __forceinline__ void evb_kick_split(const BatchArgs& A, double* S, const double* PL,
+                                               int tid, bool& nan) {
+  constexpr int JS = (M + TPB - 1) / TPB;
+  const int N = A.N, D = A.D, LD = A.LD;
+  const int b = tid / TPB, r = tid - (tid / TPB) * TPB;
+  const bool active = b < N;
+  const double kappa = PL[0];
+  const double* WA = PL + 1;
+  const double* cA = WA + M * M;
+  const double* WB = cA + M;
+  const double* cB = WB + M * M;
+  const double* lam = cB + M;
+  const double* cub = lam + M;
+  double xs[JS], gA[JS], gB[JS];
+  int jc[JS];
+#pragma unroll
+  for (int k = 0; k < JS; ++k) {
+    const int j = r + k * TPB;
+    jc[k] = j < M ? j : M - 1;  // clamped index: xs is 0 there, so the term vanishes
+    xs[k] = (active && j < D) ? S[b * LD + j] : 0.0;
+    gA[k] = 0.0;
+    gB[k] = 0.0;
+  }
+  double va = 0.0, vb = 0.0;
+#pragma unroll 1
+  for (int i = 0; i < M; ++i) {
+    const double* Wa = WA + i * M;
+    const double* Wb = WB + i * M;
+    double pa = 0.0, pb = 0.0;
+#pragma unroll
+    for (int k = 0; k < JS; ++k) {
+      pa += Wa[jc[k]] * xs[k];
+      pb += Wb[jc[k]] * xs[k];
+    }
+    const double ya = group_sum<TPB>(pa) - cA[i], yb = group_sum<TPB>(pb) - cB[i];
+    const double li = lam[i], ci = cub[i];
+    const double ya2 = ya * ya, yb2 = yb * yb;
+    va += (0.5 * li) * ya2 + ci * ya2 * ya;
+    vb += (0.5 * li) * yb2 + ci * yb2 * yb;
+    const double ha = li * ya + (3.0 * ci) * ya2, hb = li * yb + (3.0 * ci) * yb2;
+#pragma unroll
+    for (int k = 0; k < JS; ++k) {
+      gA[k] += Wa[jc[k]] * ha;
+      gB[k] += Wb[jc[k]] * hb;
+    }
+  }
+  const double dl = va - vb, R = sqrt(0.25 * dl * dl + kappa * kappa), f = dl / (4.0 * R);
+  const double wa = 0.5 - f, wb = 0.5 + f;
+#pragma unroll
+  for (int k = 0; k < JS; ++k) {
+    const int j = r + k * TPB;
+    if (active && j < D) {
+      const double np = S[b * LD + DT * 16 + j] - (wa * gA[k] + wb * gB[k]) * A.dt;
+      nan |= (np != np);
+      S[b * LD + DT * 16 + j] = np;
     }
   }
 }
@@ -215,10 +342,12 @@ __global__ void __launch_bounds__(kBatchWaves * 64)
   extern __shared__ double S[];
   __shared__ int s_flag;
   constexpr int W = kBatchWaves, DC = DT * 16;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, m = lane & 15, g = lane >> 4;
+  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (SGPR)
   const int poly = A.first + blockIdx.x;
   const int N = A.N, NPAD = A.NPAD, D = A.D, LD = A.LD, ndim = A.ndim, natom = A.natom;
   double* MT = S + NPAD * LD;
+  double* PL = MT + NPAD * kMODE;  // EVB matrices (evb_eval_at layout), KIND == PES_EVB only
   int64_t* pi = A.pi64 + (size_t)poly * kPI64;
   double* st = A.state + pi[PI_STATE_OFF];
   const double* pa = A.pdata + (size_t)poly * 3 * DC;
@@ -236,6 +365,8 @@ __global__ void __launch_bounds__(kBatchWaves * 64)
     S[i * LD + c] = v;
   }
   for (int e = tid; e < NPAD * kMODE; e += W * 64) MT[e] = A.modes[e];
+  if constexpr (KIND == PES_EVB)
+    for (int e = tid; e < evb_dev_size<MAXD>(); e += W * 64) PL[e] = A.pes.dp[e];
   if (tid == 0) s_flag = 0;
   double tisum = 0.0;
   int64_t ticnt = 0, nre = 0;
@@ -289,7 +420,19 @@ __global__ void __launch_bounds__(kBatchWaves * 64)
     for (int k = 0; k < nph; ++k) {
       const int p = ph[k];
       if (p == PH_FWD_PILE || (p == PH_FWD_A && nbwd == 1)) {
-        pes_kick<KIND, MAXD, DT>(A, S, tid, nan);  // B(dt)
+        if constexpr (KIND == PES_EVB) {  // B(dt)
+          const int tpb = evb_kick_tpb(N);
+          if (tpb == 8)
+            evb_kick_split<MAXD, DT, 8>(A, S, PL, tid, nan);
+          else if (tpb == 4)
+            evb_kick_split<MAXD, DT, 4>(A, S, PL, tid, nan);
+          else if (tpb == 2)
+            evb_kick_split<MAXD, DT, 2>(A, S, PL, tid, nan);
+          else
+            pes_kick<KIND, MAXD, DT>(A, S, PL, tid, nan);
+        } else {
+          pes_kick<KIND, MAXD, DT>(A, S, PL, tid, nan);
+        }
         __syncthreads();
       } else if (p == PH_RESAMPLE || p == PH_REINIT) {
         write_resample<DT>(A, S, MT, tid, noff, strm, cur);
@@ -384,7 +527,7 @@ int launch_batch_kind(const BatchArgs& a, int thermostat, int npoly, hipStream_t
   constexpr int W = kBatchWaves;
   const int units = (a.NPAD >> 5) * DT;
   const int upw = (units + W - 1) / W;
-  const size_t lds = batch_lds_bytes(a.NPAD, DT);
+  const size_t lds = batch_lds_bytes(a.NPAD, DT, KIND == PES_EVB ? evb_dev_size<MAXD>() : 0);
   auto go = [&](auto kern) -> int {
     RP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
@@ -394,6 +537,7 @@ int launch_batch_kind(const BatchArgs& a, int thermostat, int npoly, hipStream_t
   };
   if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1>);
   if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2>);
+  if (upw <= 3) return go(batch_kernel<KIND, MAXD, DT, 3>);
   if (upw <= 4) return go(batch_kernel<KIND, MAXD, DT, 4>);
   return set_error(RP_ERR_ARG, "batch: %d MFMA units per wave is more than supported", upw);
 }

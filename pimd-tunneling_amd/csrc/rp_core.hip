@@ -241,32 +241,38 @@ extern "C" int rp_pes_create(int kind, int ndim, int natom, const double* dp, in
   if (ip && nip > 0) P->ip.assign(ip, ip + nip);
   std::vector<double> dev = P->dp;
   if (kind == PES_EVB) {
-    // API {kappa, Q, lam, cub, xa} -> device {kappa, QA, xa, QB, xb, lam, cub}
-    // (Pi x)_j = x_{pi(j)}, pi(a*nd+d) = perm[a]*nd+d; QB(k,i) = Q(pi^-1(k), i), xb_k = xa_{pi^-1(k)}
+    // API {kappa, Q, lam, cub, xa} -> device {kappa, WtA, cwA, WtB, cwB, lam, cub} with row
+    // stride M = evb_stride(D) (rp_pes.h).  (Pi x)_j = x_{pi(j)}, pi(a*nd+d) = perm[a]*nd+d;
+    // WB(k, i) = Q(pi^-1(k), i), xb_k = xa_{pi^-1(k)}.
     const double* Q = dp + 1;
     const double* lam = Q + D * D;
     const double* cub = lam + D;
     const double* xa = cub + D;
+    const int M = evb_stride(D);
     std::vector<int> pinv(D);
     for (int a = 0; a < natom; ++a)
       for (int d = 0; d < ndim; ++d) pinv[ip[a] * ndim + d] = a * ndim + d;
-    dev.assign(1 + 2 * D * D + 4 * D, 0.0);
+    dev.assign(1 + 2 * (M * M + M) + 2 * M, 0.0);
     dev[0] = dp[0];
-    double* QA = dev.data() + 1;
-    double* xA = QA + D * D;
-    double* QB = xA + D;
-    double* xB = QB + D * D;
-    double* L = xB + D;
-    double* Cc = L + D;
-    for (int k = 0; k < D * D; ++k) QA[k] = Q[k];
-    for (int k = 0; k < D; ++k) {
-      xA[k] = xa[k];
-      xB[k] = xa[pinv[k]];
-      L[k] = lam[k];
-      Cc[k] = cub[k];
+    double* WA = dev.data() + 1;
+    double* cA = WA + M * M;
+    double* WB = cA + M;
+    double* cB = WB + M * M;
+    double* L = cB + M;
+    double* Cc = L + M;
+    for (int i = 0; i < D; ++i) {
+      double sa = 0.0, sb = 0.0;
+      for (int j = 0; j < D; ++j) {
+        WA[i * M + j] = Q[j + D * i];
+        WB[i * M + j] = Q[pinv[j] + D * i];
+        sa += Q[j + D * i] * xa[j];
+        sb += Q[pinv[j] + D * i] * xa[pinv[j]];
+      }
+      cA[i] = sa;
+      cB[i] = sb;
+      L[i] = lam[i];
+      Cc[i] = cub[i];
     }
-    for (int i = 0; i < D; ++i)
-      for (int k = 0; k < D; ++k) QB[k + D * i] = Q[pinv[k] + D * i];
   }
   RP_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
   if ((rc = P->d_dp.alloc(dev.size())) || (rc = P->d_dp.upload(dev.data(), dev.size(), P->stream))) {

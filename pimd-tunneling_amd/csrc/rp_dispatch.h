@@ -44,6 +44,7 @@ int dispatch_pes(const rp_pes* pes, F&& f) {
       break;
     case PES_EVB:
       if (D <= 16) return f.template operator()<PES_EVB, 16>();
+      if (D == 27) return f.template operator()<PES_EVB, 27>();
       if (D <= 32) return f.template operator()<PES_EVB, 32>();
       break;
     case PES_WATER:

@@ -98,4 +98,6 @@ int launch_pes_hvp(const rp_pes* pes, int nbead, int ld, const double* x, const 
                    double* hv, hipStream_t s);
 // MAXD bucket a potential needs (16, 32, 64) or 0 when unsupported
 int pes_maxd(const rp_pes* pes);
+// row stride of the device EVB layout (the MAXD bucket dispatch_pes picks)
+inline int evb_stride(int D) { return D <= 16 ? 16 : D == 27 ? 27 : 32; }
 }  // namespace rp

@@ -10,6 +10,7 @@
 //   EVB       dp = {kappa, Q[D*D], lam[D], cub[D], xa[D]}, ip = perm[natom]
 //             VA = y.Lam.y/2 + sum c y^3, y = Q^T (x - xa); VB(x) = VA(Pi x);
 //             V = (VA+VB)/2 - sqrt((VA-VB)^2/4 + kappa^2)
+//             (expanded on the device to the padded layout of evb_eval_at)
 //   WATER     dp = {De, alpha, re, kbend, theta0, eps, sigma, qH}; atoms O,H,H per
 //             molecule; Morse OH + harmonic HOH bend + LJ O-O + charges (qO = -2 qH)
 //   HARMONIC  dp = {k[D], c[D]}
@@ -67,61 +68,64 @@ __device__ __forceinline__ S harmonic_eval(const PesDev& P, const S* x, S* g) {
   return v;
 }
 
-// One harmonic+cubic well and its gradient: VW = y.Lam.y/2 + sum c y^3 with
-// y = W^T (x - xw).  Well A uses (Q, xa); well B = VA(Pi x) is the same form with
-// rows of Q permuted, W = Pi^T Q and xw = Pi^T xa (expanded on the host at creation).
-template <int MAXD, class S>
-__device__ __forceinline__ S evb_well(int D, const double* __restrict__ W,
-                                      const double* __restrict__ xw,
+// One harmonic+cubic well: VW = y.Lam.y/2 + sum c y^3 with y = W^T (x - xw), and its
+// gradient W (Lam y + 3 c y^2).  Device layout (expanded on the host at creation, rows
+// zero-padded to the compile-time stride M so no loop needs a bound check):
+//   Wt[i*M + j] = W(j, i)   (row i = column i of W),   cw[i] = sum_j W(j, i) xw_j
+// so y_i = Wt_i . x - cw_i.  Well A uses (Q, xa); well B = VA(Pi x) is the same form with
+// the rows of Q permuted (W = Pi^T Q) and xw = Pi^T xa.  Padded rows have lam = c = 0.
+template <int M, class S>
+__device__ __forceinline__ S evb_well(const double* __restrict__ Wt, const double* __restrict__ cw,
                                       const double* __restrict__ lam,
                                       const double* __restrict__ cub, const S* x, S* gx) {
-  S u[MAXD];
 #pragma unroll
-  for (int j = 0; j < MAXD; ++j) {
-    u[j] = 0.0;
-    gx[j] = 0.0;
-    if (j < D) u[j] = x[j] - xw[j];
-  }
+  for (int j = 0; j < M; ++j) gx[j] = 0.0;
   S vw = 0.0;
 #pragma unroll 1
-  for (int i = 0; i < D; ++i) {
-    const double* Wi = W + D * i;  // column i
-    S y = 0.0;
+  for (int i = 0; i < M; ++i) {
+    const double* Wi = Wt + i * M;
+    S y = -cw[i];
 #pragma unroll
-    for (int j = 0; j < MAXD; ++j)
-      if (j < D) y += Wi[j] * u[j];
+    for (int j = 0; j < M; ++j) y += Wi[j] * x[j];
+    const double li = lam[i], ci = cub[i];
     S y2 = y * y;
-    vw += (0.5 * lam[i]) * y2 + cub[i] * y2 * y;
-    S h = lam[i] * y + (3.0 * cub[i]) * y2;
+    vw += (0.5 * li) * y2 + ci * y2 * y;
+    S h = li * y + (3.0 * ci) * y2;
 #pragma unroll
-    for (int j = 0; j < MAXD; ++j)
-      if (j < D) gx[j] += Wi[j] * h;
+    for (int j = 0; j < M; ++j) gx[j] += Wi[j] * h;
   }
   return vw;
 }
 
-// Device layout of the EVB parameters (expanded from the API layout on the host):
-//   {kappa, QA[D*D], xa[D], QB[D*D], xb[D], lam[D], cub[D]}
-template <int MAXD, class S>
-__device__ __forceinline__ S evb_eval(const PesDev& P, const S* x, S* g) {
-  const int D = P.ndof;
-  const double kappa = P.dp[0];
-  const double* QA = P.dp + 1;
-  const double* xa = QA + D * D;
-  const double* QB = xa + D;
-  const double* xb = QB + D * D;
-  const double* lam = xb + D;
-  const double* cub = lam + D;
-  S gb[MAXD];
-  S va = evb_well<MAXD>(D, QA, xa, lam, cub, x, g);
-  S vb = evb_well<MAXD>(D, QB, xb, lam, cub, x, gb);
+// Device layout of the EVB parameters with row stride M (see evb_well):
+//   {kappa, WtA[M*M], cwA[M], WtB[M*M], cwB[M], lam[M], cub[M]}
+template <int M>
+__host__ __device__ constexpr int evb_dev_size() { return 1 + 2 * (M * M + M) + 2 * M; }
+
+template <int M, class S>
+__device__ __forceinline__ S evb_eval_at(const double* __restrict__ dp, const S* x, S* g) {
+  const double kappa = dp[0];
+  const double* WA = dp + 1;
+  const double* cA = WA + M * M;
+  const double* WB = cA + M;
+  const double* cB = WB + M * M;
+  const double* lam = cB + M;
+  const double* cub = lam + M;
+  S gb[M];
+  S va = evb_well<M>(WA, cA, lam, cub, x, g);
+  S vb = evb_well<M>(WB, cB, lam, cub, x, gb);
   S dl = va - vb;
   S R = sqrt(0.25 * dl * dl + kappa * kappa);
   S f = dl / (4.0 * R);
+  S wa = 0.5 - f, wb = 0.5 + f;
 #pragma unroll
-  for (int j = 0; j < MAXD; ++j)
-    if (j < D) g[j] = 0.5 * (g[j] + gb[j]) - f * (g[j] - gb[j]);
+  for (int j = 0; j < M; ++j) g[j] = wa * g[j] + wb * gb[j];
   return 0.5 * (va + vb) - R;
+}
+
+template <int M, class S>
+__device__ __forceinline__ S evb_eval(const PesDev& P, const S* x, S* g) {
+  return evb_eval_at<M>(P.dp, x, g);
 }
 
 template <int NMOL, class S>

@@ -1,0 +1,187 @@
+"""Thermodynamic-integration driver arithmetic (host side of pimd_par.f90).
+
+Gauss-Legendre nodes (verletmodule.f90:124-160), the task table and the final
+quadrature / error / q/q0 (pimd_par.f90:243-308, 321-424), and the splitting
+formulas (rpi_ser.f90:368-381; multiwell_waterdimer.py:6-16).  The trajectories
+themselves run in the batched engine (``Batch``); this module only places the
+lambda nodes and reduces the integrands, as the reference's driver does.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+PI_REF = 3.14159265358979  # instantonmod.f90:4
+
+
+def gauleg(x1, x2, n):
+    """Gauss-Legendre nodes and weights on [x1, x2] (verletmodule.f90:124-160)."""
+    EPS = 3.0e-14
+    x = np.zeros(n)
+    w = np.zeros(n)
+    m = (n + 1) // 2
+    xm, xl = 0.5 * (x2 + x1), 0.5 * (x2 - x1)
+    for i in range(1, m + 1):
+        z = math.cos(PI_REF * (i - 0.25) / (n + 0.5))
+        while True:
+            p1, p2 = 1.0, 0.0
+            for j in range(1, n + 1):
+                p3 = p2
+                p2 = p1
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j
+            pp = n * (z * p1 - p2) / (z * z - 1.0)
+            z1 = z
+            z = z1 - p1 / pp
+            if abs(z - z1) <= EPS:
+                break
+        x[i - 1] = xm - xl * z
+        x[n - i] = xm + xl * z
+        w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp)
+        w[n - i] = w[i - 1]
+    return x, w
+
+
+def task_table(nintegral, nrep):
+    """lambda-major task order of pimd_par.f90:245-255: task k = nrep*(i-1) + j."""
+    return [(i, j) for i in range(nintegral) for j in range(nrep)]
+
+
+def skip_task(endpoint):
+    """pimd_par.f90:322-325: a node whose endpoint is within 1e-10 of the origin in every
+    coordinate is not run and contributes integrand 0."""
+    return bool(np.all(np.abs(endpoint) < 1e-10))
+
+
+def quadrature(integrands, weights, nrep, betan):
+    """pimd_par.f90:397-424: mean and population variance over the nrep replicas of each
+    node, Delta A = sum w_i mean_i, sigma_A^2 = sum w_i^2 var_i, q/q0 = exp(-Delta A betan)
+    and its error betan sqrt(sigma_A^2/nrep) q/q0."""
+    I = np.asarray(integrands, dtype=np.float64).reshape(len(weights), nrep)
+    mean = I.sum(axis=1) / nrep
+    var = (I ** 2).sum(axis=1) / nrep - mean ** 2
+    answer = float(np.sum(np.where(np.isnan(mean), 0.0, np.asarray(weights) * mean)))
+    sigmaA = float(np.sum(var * np.asarray(weights) ** 2))
+    q = math.exp(-answer * betan)
+    return {"deltaA": answer, "sigmaA": sigmaA, "q_ratio": q,
+            "q_err": betan * math.sqrt(sigmaA / nrep) * q}
+
+
+def splitting_two_beta(I1, I2, beta1, beta2):
+    """multiwell_waterdimer.py:6-10: Delta = 2 (artanh I2 - artanh I1) / (beta2 - beta1)."""
+    return 2.0 * (math.atanh(I2) - math.atanh(I1)) / (beta2 - beta1)
+
+
+def splitting_single_beta(I, beta):
+    """Two-level limit of the above at one temperature: Delta = 2 artanh(I) / beta."""
+    return 2.0 * math.atanh(I) / beta
+
+
+def rpi_splitting(betan, skink, lndetj, lndetj0):
+    """rpi_ser.f90:368-381 -> (phi, theta, delta) with the reference's pi literal."""
+    phi = math.exp(0.5 * (lndetj - lndetj0))
+    theta = betan * math.exp(-skink) * math.sqrt(skink / (2.0 * PI_REF)) / phi
+    return phi, theta, 2.0 * theta / betan
+
+
+def run_ti(batch_factory, spec, n, beta, nintegral, nrep, nsteps, imin, path=None, **kw):
+    """PIMD-TI on the batched engine with the reference driver's semantics.
+
+    batch_factory(polymers) -> Batch.  Every (node, replica) task of pimd_par.f90 becomes
+    one polymer; the integrand is dHdr/betan^2 with dHdr the running TI mean
+    (pimd_par.f90:379).  Returns the quadrature dict plus the raw integrands."""
+    from .specs import arc_lampath, linear_path
+
+    betan = beta / (n + 1)
+    if path is None:
+        lampath, pth = linear_path(spec.well1, spec.well2, npath=2)
+    else:
+        pth = np.asarray(path, dtype=np.float64)
+        lampath = arc_lampath(pth)
+    xi, w = gauleg(0.0, 1.0, nintegral)
+    from .specs import PesSpec  # noqa: F401
+
+    # endpoints b(xi) and db/dxi from the path spline (pimd_par.f90:214-221)
+    from ._lib import lib  # noqa: F401
+    ends, grads = _spline_nodes(lampath, pth, xi)
+    start = pth[:, 0]
+    polymers, keep = [], []
+    for k, (i, j) in enumerate(task_table(nintegral, nrep)):
+        if skip_task(ends[i]):
+            continue
+        polymers.append(dict(n=n, betan=betan, xi=float(xi[i]), a=start, b=ends[i], dbdl=grads[i],
+                             stream=k))
+        keep.append(k)
+    integrands = np.zeros(nintegral * nrep)
+    if polymers:
+        b = batch_factory(polymers)
+        b.set_path(lampath, pth)
+        b.init_path()
+        b.run(nsteps, imin=imin)
+        sums, cnt, _ = b.ti_readout()
+        for k, s in zip(keep, sums):
+            integrands[k] = s / (nsteps - imin) / betan ** 2
+    res = quadrature(integrands, w, nrep, betan)
+    res["integrands"] = integrands
+    res["nodes"] = xi
+    res["weights"] = w
+    return res
+
+
+def _spline_nodes(lampath, path, xi):
+    """splint / splin_grad of every dof at the nodes (instantonmod.f90:499-561)."""
+    npath = len(lampath)
+    ndof = path.shape[0]
+    y2 = np.zeros_like(path)
+    for d in range(ndof):
+        y2[d] = _natural_spline(lampath, path[d])
+    ends = np.zeros((len(xi), ndof))
+    grads = np.zeros((len(xi), ndof))
+    for i, x in enumerate(xi):
+        klo = max(min(_locate(lampath, x), npath - 1), 1) - 1
+        khi = klo + 1
+        h = lampath[khi] - lampath[klo]
+        a = (lampath[khi] - x) / h
+        b = (x - lampath[klo]) / h
+        ends[i] = a * path[:, klo] + b * path[:, khi] + ((a ** 3 - a) * y2[:, klo] + (b ** 3 - b) * y2[:, khi]) * h * h / 6.0
+        grads[i] = (path[:, khi] - path[:, klo]) / h + ((1.0 - 3.0 * a * a) * y2[:, klo] + (3.0 * b * b - 1.0) * y2[:, khi]) * h / 6.0
+    return ends, grads
+
+
+def _locate(xx, x):
+    n = len(xx)
+    ascnd = xx[-1] >= xx[0]
+    jl, ju = 0, n + 1
+    while ju - jl > 1:
+        jm = (ju + jl) // 2
+        if ascnd == (x >= xx[jm - 1]):
+            jl = jm
+        else:
+            ju = jm
+    if x == xx[0]:
+        return 1
+    if x == xx[-1]:
+        return n - 1
+    return jl
+
+
+def _natural_spline(x, y):
+    n = len(x)
+    a, b, c, r, gam, y2 = (np.zeros(n) for _ in range(6))
+    c[: n - 1] = x[1:] - x[:-1]
+    r[: n - 1] = 6.0 * ((y[1:] - y[:-1]) / c[: n - 1])
+    r[1 : n - 1] = r[1 : n - 1] - r[: n - 2].copy()
+    a[1 : n - 1] = c[: n - 2]
+    b[1 : n - 1] = 2.0 * (c[1 : n - 1] + a[1 : n - 1])
+    b[0] = b[n - 1] = 1.0
+    r[0] = c[0] = 0.0
+    r[n - 1] = a[n - 1] = 0.0
+    bet = b[0]
+    y2[0] = r[0] / bet
+    for j in range(1, n):
+        gam[j] = c[j - 1] / bet
+        bet = b[j] - a[j] * gam[j]
+        y2[j] = (r[j] - a[j] * y2[j - 1]) / bet
+    for j in range(n - 2, -1, -1):
+        y2[j] = y2[j] - gam[j + 1] * y2[j + 1]
+    return y2
