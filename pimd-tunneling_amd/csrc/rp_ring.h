@@ -1,0 +1,17 @@
+// rp_ring.h -- internal interface between the ring kernels and the instanton driver.
+#pragma once
+#include <vector>
+
+#include "rp_internal.h"
+
+namespace rp {
+int ring_eval(rp_ring* r, bool want_grad);            // energies (+ adjoint gradients) on device
+int ring_upload_x(rp_ring* r, const double* x);
+int ring_download(rp_ring* r, double* e, double* g);  // synchronises the ring's stream
+int ring_tot(const rp_ring* r);
+int ring_nchain(const rp_ring* r);
+int ring_D(const rp_ring* r);
+const std::vector<int>& ring_n(const rp_ring* r);
+const std::vector<int64_t>& ring_off(const rp_ring* r);
+int ring_lowest_eigen(rp_ring* r, double* eta);
+}  // namespace rp

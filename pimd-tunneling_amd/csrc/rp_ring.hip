@@ -1,0 +1,670 @@
+// rp_ring.hip -- ring-polymer action, adjoint gradient, Hessian and log-determinant
+// for batches of open fixed-end chains of uneven length (instantonmod.f90:17-217,
+// 782-827), the device side of the instanton search.
+//
+// Layout: chains concatenated, chain c at offset off[c] = ndof * sum_{c'<c} n_c', each
+// chain (n, ndim, natom) column-major: x[off + dof*n + bead].  One workgroup per chain.
+#include <cmath>
+#include <cstring>
+
+#include "rp_dispatch.h"
+#include "rp_ring.h"
+#include <algorithm>
+
+using namespace rp;
+
+namespace {
+constexpr int kRingThreads = 256;
+
+struct RingArgs {
+  PesDev pes;
+  int nchain, D, ndim, natom, fixedends;
+  const int* n;
+  const int64_t* off;      // per chain, in doubles
+  const int64_t* boff;     // per chain, first global bead index
+  const double* betan;
+  const double* ab;        // per chain [a | b], D each
+  const double* massd;     // per dof
+};
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = kRingThreads / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// U_M and its adjoint gradient (UMforceenergy, instantonmod.f90:93-151):
+// U = sum_i V(x_i) + sum_{i<n} (m/2bn^2)|x_{i+1}-x_i|^2 [+ (m/2bn^2)(|x_1-a|^2 + |b-x_n|^2)]
+template <int KIND, int MAXD>
+__global__ void __launch_bounds__(kRingThreads) ring_um_kernel(RingArgs A, const double* __restrict__ x,
+                                                               double* __restrict__ grad,
+                                                               double* __restrict__ energy) {
+  __shared__ double red[kRingThreads];
+  const int c = blockIdx.x;
+  const int n = A.n[c], D = A.D;
+  const double bn2 = A.betan[c] * A.betan[c];
+  const double* xc = x + A.off[c];
+  const double* a = A.ab + (size_t)c * 2 * D;
+  const double* b = a + D;
+  double e = 0.0;
+  for (int i = threadIdx.x; i < n; i += kRingThreads) {
+    double xs[MAXD], gs[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) xs[d] = (d < D) ? xc[(size_t)d * n + i] : 0.0;
+    e += eval_kind<KIND, MAXD>(A.pes, xs, gs);
+    for (int d = 0; d < D; ++d) {
+      const double m = A.massd[d], xi = xs[d];
+      const double* xd = xc + (size_t)d * n;
+      if (i < n - 1) {
+        const double dx = xd[i + 1] - xi;
+        e += (0.5 * m / bn2) * dx * dx;
+      }
+      double sg;
+      if (i == 0)
+        sg = A.fixedends ? m * (2.0 * xi - a[d] - xd[1]) / bn2 : m * (xi - xd[1]) / bn2;
+      else if (i == n - 1)
+        sg = A.fixedends ? m * (2.0 * xi - xd[n - 2] - b[d]) / bn2 : m * (xi - xd[n - 2]) / bn2;
+      else
+        sg = m * (2.0 * xi - xd[i - 1] - xd[i + 1]) / bn2;
+      if (A.fixedends && i == 0) e += (0.5 * m / bn2) * (xi - a[d]) * (xi - a[d]);
+      if (A.fixedends && i == n - 1) e += (0.5 * m / bn2) * (b[d] - xi) * (b[d] - xi);
+      if (grad) grad[A.off[c] + (size_t)d * n + i] = sg + gs[d];
+    }
+  }
+  const double tot = block_sum(e, red);
+  if (threadIdx.x == 0) energy[c] = tot;
+}
+
+// Hessian-vector product of U_M: springs exactly, potential by forward-over-reverse.
+template <int KIND, int MAXD>
+__global__ void __launch_bounds__(kRingThreads) ring_hvp_kernel(RingArgs A, const double* __restrict__ x,
+                                                                const double* __restrict__ v,
+                                                                double* __restrict__ hv) {
+  const int c = blockIdx.x;
+  const int n = A.n[c], D = A.D;
+  const double bn2 = A.betan[c] * A.betan[c];
+  const int64_t o = A.off[c];
+  for (int i = threadIdx.x; i < n; i += kRingThreads) {
+    Dual xs[MAXD], gs[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d)
+      xs[d] = (d < D) ? Dual(x[o + (size_t)d * n + i], v[o + (size_t)d * n + i]) : Dual(0.0);
+    (void)eval_kind<KIND, MAXD>(A.pes, xs, gs);
+    for (int d = 0; d < D; ++d) {
+      const double m = A.massd[d];
+      const double* vd = v + o + (size_t)d * n;
+      double s;
+      if (i == 0)
+        s = A.fixedends ? 2.0 * vd[0] - (n > 1 ? vd[1] : 0.0) : vd[0] - (n > 1 ? vd[1] : 0.0);
+      else if (i == n - 1)
+        s = A.fixedends ? 2.0 * vd[i] - vd[i - 1] : vd[i] - vd[i - 1];
+      else
+        s = 2.0 * vd[i] - vd[i - 1] - vd[i + 1];
+      hv[o + (size_t)d * n + i] = m * s / bn2 + gs[d].d;
+    }
+  }
+}
+
+// Potential Hessian blocks of every bead from D forward-over-reverse sweeps:
+// blocks[gbead][i][k] = d grad_k / d x_i.  Thread (bead, i).
+template <int KIND, int MAXD>
+__global__ void ring_hblock_kernel(RingArgs A, const double* __restrict__ x, int single_first,
+                                   double* __restrict__ blocks) {
+  const int c = blockIdx.y;
+  const int n = A.n[c], D = A.D;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int bead = t / D, i = t - bead * D;
+  if (bead >= n) return;
+  const int src = single_first ? 0 : bead;  // singlewell: bead 1's Hessian everywhere
+  const int64_t o = A.off[c];
+  Dual xs[MAXD], gs[MAXD];
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d)
+    xs[d] = (d < D) ? Dual(x[o + (size_t)d * n + src], d == i ? 1.0 : 0.0) : Dual(0.0);
+  (void)eval_kind<KIND, MAXD>(A.pes, xs, gs);
+  double* blk = blocks + ((size_t)(A.boff[c] + bead) * D + i) * D;
+  for (int k = 0; k < D; ++k) blk[k] = gs[k].d;
+}
+
+// UMhessian's band (instantonmod.f90:155-217) from the blocks of one chain: LAPACK lower
+// band storage band[r*tot + col] = A(col + r, col), r = 0..D.  The inter-bead coupling
+// -1/bn^2 is written only for beads i > 1 (1-based), as the reference does (:203).
+__global__ void ring_band_kernel(int n, int D, double bn2, const double* __restrict__ massd,
+                                 const double* __restrict__ blocks, double* __restrict__ band) {
+  const int64_t tot = (int64_t)n * D;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (D + 1) * tot) return;
+  const int r = (int)(t / tot);
+  const int64_t col = t - (int64_t)r * tot;
+  const int bead = (int)(col / D), i1 = (int)(col - (int64_t)bead * D);
+  double v = 0.0;
+  if (r == 0) {
+    v = 2.0 / bn2 + blocks[((size_t)bead * D + i1) * D + i1] / sqrt(massd[i1] * massd[i1]);
+  } else if (r == D) {
+    if (bead > 0) v = -1.0 / bn2;  // beyond the matrix for the last bead; never read
+  } else if (i1 + r < D) {
+    const int i2 = i1 + r;
+    // hess(j2,k2,j1,k1) = d grad(idof1) / d x(idof2)
+    v = blocks[((size_t)bead * D + i2) * D + i1] / sqrt(massd[i1] * massd[i2]);
+  }
+  band[t] = v;
+}
+
+// dU_M/dtheta = sum over beads of dV/dtheta (water: theta = qH, De), one block per chain.
+template <int NMOL>
+__global__ void __launch_bounds__(kRingThreads) ring_param_grad_kernel(RingArgs A,
+                                                                       const double* __restrict__ x,
+                                                                       double* __restrict__ out) {
+  __shared__ double red[kRingThreads];
+  const int c = blockIdx.x, n = A.n[c];
+  const double* xc = x + A.off[c];
+  double p0 = 0.0, p1 = 0.0;
+  for (int i = threadIdx.x; i < n; i += kRingThreads) {
+    double xs[9 * NMOL], gs[9 * NMOL], pg[2];
+#pragma unroll
+    for (int d = 0; d < 9 * NMOL; ++d) xs[d] = xc[(size_t)d * n + i];
+    (void)water_eval<NMOL>(A.pes, xs, gs, pg);
+    p0 += pg[0];
+    p1 += pg[1];
+  }
+  p0 = block_sum(p0, red);
+  p1 = block_sum(p1, red);
+  if (threadIdx.x == 0) {
+    out[2 * c] = p0;
+    out[2 * c + 1] = p1;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- log-determinant
+// Block LDL^T of the mass-weighted ring Hessian (the reference's band matrix):
+//   A_i = (2/bn^2) I + M^-1/2 H_i M^-1/2,   coupling -(1/bn^2) I between beads (i, i+1)
+//   for i >= 1 (0-based; bead 0 is not coupled to bead 1, instantonmod.f90:203).
+//   S_0 = A_0,  S_i = A_i - (1/bn^4) S_{i-1}^-1 when (i-1, i) is coupled, else A_i.
+// log|det| = sum log|det S_i|, inertia = sum inertia(S_i) (Haynsworth).  The S_i^-1 are
+// kept for inverse iteration on the eigenvalue nearest zero.
+namespace {
+constexpr int kMaxD = 56;  // ndof bound of the log-determinant workspace (LDS)
+
+// LDL^T of the D x D symmetric matrix in Sm (LDS, row stride D) and its inverse into Si;
+// returns log|det| and the count of negative pivots through *lg / *neg (thread 0 valid).
+__device__ void ldlt_inverse(int D, double* Sm, double* Si, double* dg, double& lg, int& neg) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  // right-looking LDL^T, L stored strictly below the diagonal of Sm
+  for (int j = 0; j < D; ++j) {
+    const double dj = Sm[j * D + j];
+    __syncthreads();
+    for (int r = j + 1 + tid; r < D; r += nt) Sm[r * D + j] = Sm[r * D + j] / dj;
+    __syncthreads();
+    const int m = D - j - 1;
+    for (int e = tid; e < m * m; e += nt) {
+      const int r = j + 1 + e / m, cc = j + 1 + e % m;
+      if (cc <= r) Sm[r * D + cc] -= Sm[r * D + j] * dj * Sm[cc * D + j];
+    }
+    if (tid == 0) dg[j] = dj;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double l = 0.0;
+    int ng = 0;
+    for (int j = 0; j < D; ++j) {
+      l += log(fabs(dg[j]));
+      ng += dg[j] < 0.0;
+    }
+    lg = l;
+    neg = ng;
+  }
+  // inverse: solve L Y = I, Y /= d, L^T X = Y; column e handled by thread e
+  for (int e = tid; e < D; e += nt) {
+    for (int r = 0; r < D; ++r) {
+      double s = (r == e) ? 1.0 : 0.0;
+      for (int k = 0; k < r; ++k) s -= Sm[r * D + k] * Si[k * D + e];
+      Si[r * D + e] = s;
+    }
+    for (int r = 0; r < D; ++r) Si[r * D + e] /= dg[r];
+    for (int r = D - 1; r >= 0; --r) {
+      double s = Si[r * D + e];
+      for (int k = r + 1; k < D; ++k) s -= Sm[k * D + r] * Si[k * D + e];
+      Si[r * D + e] = s;
+    }
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kRingThreads) ring_logdet_kernel(
+    int D, const int* __restrict__ nn, const int64_t* __restrict__ boff,
+    const double* __restrict__ betan, const double* __restrict__ massd,
+    const double* __restrict__ blocks, const double* __restrict__ shift, double* __restrict__ sinv,
+    double* __restrict__ logdet, int* __restrict__ nneg) {
+  __shared__ double Sm[kMaxD * kMaxD], Si[kMaxD * kMaxD], dg[kMaxD];
+  const int c = blockIdx.x, n = nn[c], tid = threadIdx.x;
+  const double bn2 = betan[c] * betan[c], c2 = 1.0 / (bn2 * bn2);
+  const double sig = shift ? shift[c] : 0.0;  // factor H - sig I
+  double lg_tot = 0.0;
+  int neg_tot = 0;
+  for (int i = 0; i < n; ++i) {
+    const double* blk = blocks + (size_t)(boff[c] + i) * D * D;
+    const double* prev = sinv + (size_t)(boff[c] + i - 1) * D * D;
+    const bool coupled = i >= 2;  // pair (i-1, i) with i-1 >= 1
+    for (int e = tid; e < D * D; e += blockDim.x) {
+      const int r = e / D, cc = e - r * D;
+      // A_i(r, cc) = 2/bn^2 delta + H(cc -> r) / sqrt(m_r m_cc); blocks are d grad_k / d x_i
+      double v = blk[r * D + cc] / sqrt(massd[r] * massd[cc]);  // lower triangle convention
+      if (r == cc) v += 2.0 / bn2 - sig;
+      if (coupled) v -= c2 * prev[e];
+      Sm[e] = v;
+    }
+    __syncthreads();
+    double lg;
+    int ng;
+    ldlt_inverse(D, Sm, Si, dg, lg, ng);
+    if (tid == 0) {
+      lg_tot += lg;
+      neg_tot += ng;
+    }
+    double* out = sinv + (size_t)(boff[c] + i) * D * D;
+    for (int e = tid; e < D * D; e += blockDim.x) out[e] = Si[e];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    logdet[c] = lg_tot;
+    nneg[c] = neg_tot;
+  }
+}
+
+// y = H^-1 v through the stored block factors (block forward / back substitution).
+__device__ void block_solve(int D, int n, double c, const double* __restrict__ sinv,
+                            double* w /* global, n*D: in v, out y */, double* tmp /* LDS D */) {
+  const int tid = threadIdx.x;
+  // forward: w_i += c S_{i-1}^-1 w_{i-1} for coupled (i-1, i), i >= 2
+  for (int i = 2; i < n; ++i) {
+    const double* Sp = sinv + (size_t)(i - 1) * D * D;
+    for (int r = tid; r < D; r += blockDim.x) {
+      double s = 0.0;
+      for (int k = 0; k < D; ++k) s += Sp[r * D + k] * w[(size_t)(i - 1) * D + k];
+      tmp[r] = s;
+    }
+    __syncthreads();
+    for (int r = tid; r < D; r += blockDim.x) w[(size_t)i * D + r] += c * tmp[r];
+    __syncthreads();
+  }
+  // back: y_{n-1} = S^-1 w_{n-1}; y_i = S_i^-1 (w_i + c y_{i+1}) when (i, i+1) coupled
+  for (int i = n - 1; i >= 0; --i) {
+    const double* Sc = sinv + (size_t)i * D * D;
+    const bool coupled = (i + 1 < n) && (i >= 1);
+    for (int r = tid; r < D; r += blockDim.x) {
+      double s = 0.0;
+      for (int k = 0; k < D; ++k) {
+        double rhs = w[(size_t)i * D + k];
+        if (coupled) rhs += c * w[(size_t)(i + 1) * D + k];
+        s += Sc[r * D + k] * rhs;
+      }
+      tmp[r] = s;
+    }
+    __syncthreads();
+    for (int r = tid; r < D; r += blockDim.x) w[(size_t)i * D + r] = tmp[r];
+    __syncthreads();
+  }
+}
+
+// Inverse iteration for the eigenvalue of H nearest the factorisation shift sig:
+// v <- (H - sig)^-1 v / |.|, eta = sig + 1 / (v . (H - sig)^-1 v) for the normalised v.
+// work holds [v | y] per chain (2 n D); fresh = 1 starts from a fixed generic vector,
+// otherwise the previous v is continued (Rayleigh-quotient refinement).
+__global__ void __launch_bounds__(kRingThreads) ring_eta1_kernel(
+    int D, const int* __restrict__ nn, const int64_t* __restrict__ boff,
+    const double* __restrict__ betan, const double* __restrict__ sinv, double* __restrict__ work,
+    const double* __restrict__ shift, int iters, int fresh, double* __restrict__ eta1) {
+  __shared__ double red[kRingThreads];
+  __shared__ double tmp[kMaxD];
+  const int cix = blockIdx.x, n = nn[cix], tid = threadIdx.x;
+  const int64_t tot = (int64_t)n * D;
+  const double c = 1.0 / (betan[cix] * betan[cix]);
+  double* v = work + 2 * boff[cix] * D;
+  double* y = v + tot;
+  const double* S = sinv + (size_t)boff[cix] * D * D;
+  const double sig = shift ? shift[cix] : 0.0;
+  double nrm = 0.0;
+  for (int64_t e = tid; e < tot; e += blockDim.x) {
+    const double t = fresh ? 1.0 + 0.37 * (double)((e * 7919) % 101) / 101.0 : v[e];
+    v[e] = t;
+    nrm += t * t;
+  }
+  nrm = sqrt(block_sum(nrm, red));
+  for (int64_t e = tid; e < tot; e += blockDim.x) v[e] /= nrm;
+  __syncthreads();
+  double eta = 0.0, prev = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    for (int64_t e = tid; e < tot; e += blockDim.x) y[e] = v[e];
+    __syncthreads();
+    block_solve(D, n, c, S, y, tmp);
+    double mu = 0.0, yy = 0.0;
+    for (int64_t e = tid; e < tot; e += blockDim.x) {
+      mu += v[e] * y[e];
+      yy += y[e] * y[e];
+    }
+    mu = block_sum(mu, red);
+    yy = sqrt(block_sum(yy, red));
+    for (int64_t e = tid; e < tot; e += blockDim.x) v[e] = y[e] / yy;
+    __syncthreads();
+    eta = sig + 1.0 / mu;
+    if (it > 4 && fabs(eta - prev) <= 1e-15 * fabs(eta)) break;  // block-uniform
+    prev = eta;
+  }
+  if (tid == 0) eta1[cix] = eta;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host
+struct rp_ring {
+  rp_pes* pes = nullptr;
+  int nchain = 0, D = 0, ndim = 0, natom = 0, fixedends = 1;
+  std::vector<int> n;
+  std::vector<double> betan;
+  std::vector<int64_t> off, boff;
+  int64_t tot = 0, nbeads = 0;
+  hipStream_t s = nullptr;
+  DevBuf<double> d_x, d_g, d_v, d_hv, d_e, d_ab, d_betan, d_massd, d_blocks, d_sinv, d_work, d_ld,
+      d_eta;
+  DevBuf<int> d_n, d_nneg;
+  DevBuf<int64_t> d_off, d_boff;
+};
+
+static RingArgs ring_args(rp_ring* r) {
+  RingArgs a;
+  a.pes = r->pes->dev();
+  a.nchain = r->nchain;
+  a.D = r->D;
+  a.ndim = r->ndim;
+  a.natom = r->natom;
+  a.fixedends = r->fixedends;
+  a.n = r->d_n.p;
+  a.off = r->d_off.p;
+  a.boff = r->d_boff.p;
+  a.betan = r->d_betan.p;
+  a.ab = r->d_ab.p;
+  a.massd = r->d_massd.p;
+  return a;
+}
+
+namespace rp {
+int ring_eval(rp_ring* r, bool want_grad) {
+  RingArgs a = ring_args(r);
+  return dispatch_pes(r->pes, [&]<int K, int M>() -> int {
+    ring_um_kernel<K, M><<<r->nchain, kRingThreads, 0, r->s>>>(a, r->d_x.p,
+                                                              want_grad ? r->d_g.p : nullptr, r->d_e.p);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  });
+}
+int ring_tot(const rp_ring* r) { return (int)r->tot; }
+int ring_nchain(const rp_ring* r) { return r->nchain; }
+const std::vector<int>& ring_n(const rp_ring* r) { return r->n; }
+const std::vector<int64_t>& ring_off(const rp_ring* r) { return r->off; }
+int ring_D(const rp_ring* r) { return r->D; }
+int ring_upload_x(rp_ring* r, const double* x) { return r->d_x.upload(x, r->tot, r->s); }
+int ring_download(rp_ring* r, double* e, double* g) {
+  int rc;
+  if (e && (rc = r->d_e.download(e, r->nchain, r->s))) return rc;
+  if (g && (rc = r->d_g.download(g, r->tot, r->s))) return rc;
+  RP_HIP(hipStreamSynchronize(r->s));
+  return RP_OK;
+}
+}  // namespace rp
+
+extern "C" int rp_ring_create(rp_pes* pes, const double* mass, int nchain, const int* n,
+                              const double* betan, const double* a, const double* b,
+                              int fixedends, rp_ring** out) {
+  RP_REQUIRE(out, "rp_ring_create: out is NULL");
+  *out = nullptr;
+  RP_REQUIRE(pes && mass && n && betan && nchain > 0, "rp_ring_create: bad arguments");
+  RP_REQUIRE(pes->ndof <= 56, "rp_ring_create: ndof %d > 56", pes->ndof);
+  int rc = ensure_init();
+  if (rc) return rc;
+  rp_ring* r = new rp_ring();
+  r->pes = pes;
+  r->nchain = nchain;
+  r->D = pes->ndof;
+  r->ndim = pes->ndim;
+  r->natom = pes->natom;
+  r->fixedends = fixedends ? 1 : 0;
+  const int D = r->D;
+  r->n.assign(n, n + nchain);
+  r->betan.assign(betan, betan + nchain);
+  std::vector<double> ab((size_t)nchain * 2 * D, 0.0);
+  for (int c = 0; c < nchain; ++c) {
+    if (n[c] < 2) {
+      delete r;
+      return set_error(RP_ERR_ARG, "rp_ring_create: chain %d has n=%d < 2", c, n[c]);
+    }
+    r->off.push_back(r->tot);
+    r->boff.push_back(r->nbeads);
+    r->tot += (int64_t)n[c] * D;
+    r->nbeads += n[c];
+    if (a) memcpy(ab.data() + (size_t)c * 2 * D, a + (size_t)c * D, D * sizeof(double));
+    if (b) memcpy(ab.data() + (size_t)c * 2 * D + D, b + (size_t)c * D, D * sizeof(double));
+  }
+  std::vector<double> massd(D);
+  for (int at = 0; at < r->natom; ++at)
+    for (int j = 0; j < r->ndim; ++j) massd[at * r->ndim + j] = mass[at];
+  if ((rc = [&]() -> int {
+        RP_HIP(hipStreamCreateWithFlags(&r->s, hipStreamNonBlocking));
+        int q;
+        if ((q = r->d_x.alloc(r->tot)) || (q = r->d_g.alloc(r->tot)) || (q = r->d_v.alloc(r->tot)) ||
+            (q = r->d_hv.alloc(r->tot)) || (q = r->d_e.alloc(nchain)))
+          return q;
+        RP_HIP(hipMemsetAsync(r->d_x.p, 0, r->tot * sizeof(double), r->s));
+        if ((q = r->d_ab.alloc(ab.size())) || (q = r->d_ab.upload(ab.data(), ab.size(), r->s))) return q;
+        if ((q = r->d_betan.alloc(nchain)) || (q = r->d_betan.upload(betan, nchain, r->s))) return q;
+        if ((q = r->d_massd.alloc(D)) || (q = r->d_massd.upload(massd.data(), D, r->s))) return q;
+        if ((q = r->d_n.alloc(nchain)) || (q = r->d_n.upload(n, nchain, r->s))) return q;
+        if ((q = r->d_off.alloc(nchain)) || (q = r->d_off.upload(r->off.data(), nchain, r->s))) return q;
+        if ((q = r->d_boff.alloc(nchain)) || (q = r->d_boff.upload(r->boff.data(), nchain, r->s))) return q;
+        RP_HIP(hipStreamSynchronize(r->s));
+        return RP_OK;
+      }())) {
+    rp_ring_destroy(r);
+    return rc;
+  }
+  *out = r;
+  return RP_OK;
+}
+
+extern "C" int rp_ring_destroy(rp_ring* r) {
+  if (!r) return RP_OK;
+  if (r->s) (void)hipStreamSynchronize(r->s);
+  for (DevBuf<double>* b : {&r->d_x, &r->d_g, &r->d_v, &r->d_hv, &r->d_e, &r->d_ab, &r->d_betan,
+                            &r->d_massd, &r->d_blocks, &r->d_sinv, &r->d_work, &r->d_ld, &r->d_eta})
+    b->free();
+  r->d_n.free();
+  r->d_nneg.free();
+  r->d_off.free();
+  r->d_boff.free();
+  if (r->s) (void)hipStreamDestroy(r->s);
+  delete r;
+  return RP_OK;
+}
+
+extern "C" int rp_ring_set_x(rp_ring* r, const double* x) {
+  RP_REQUIRE(r && x, "rp_ring_set_x: bad arguments");
+  int rc = r->d_x.upload(x, r->tot, r->s);
+  if (rc) return rc;
+  RP_HIP(hipStreamSynchronize(r->s));
+  return RP_OK;
+}
+
+extern "C" int rp_ring_get_x(rp_ring* r, double* x) {
+  RP_REQUIRE(r && x, "rp_ring_get_x: bad arguments");
+  int rc = r->d_x.download(x, r->tot, r->s);
+  if (rc) return rc;
+  RP_HIP(hipStreamSynchronize(r->s));
+  return RP_OK;
+}
+
+extern "C" int rp_ring_um(rp_ring* r, double* energy, double* grad) {
+  RP_REQUIRE(r && energy, "rp_ring_um: bad arguments");
+  int rc = ring_eval(r, grad != nullptr);
+  if (rc) return rc;
+  return ring_download(r, energy, grad);
+}
+
+extern "C" int rp_ring_hvp(rp_ring* r, const double* v, double* hv) {
+  RP_REQUIRE(r && v && hv, "rp_ring_hvp: bad arguments");
+  int rc;
+  if ((rc = r->d_v.upload(v, r->tot, r->s))) return rc;
+  RingArgs a = ring_args(r);
+  rc = dispatch_pes(r->pes, [&]<int K, int M>() -> int {
+    ring_hvp_kernel<K, M><<<r->nchain, kRingThreads, 0, r->s>>>(a, r->d_x.p, r->d_v.p, r->d_hv.p);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  });
+  if (rc) return rc;
+  if ((rc = r->d_hv.download(hv, r->tot, r->s))) return rc;
+  RP_HIP(hipStreamSynchronize(r->s));
+  return RP_OK;
+}
+
+static int ring_blocks(rp_ring* r, int singlewell) {
+  int rc;
+  const size_t nb = (size_t)r->nbeads * r->D * r->D;
+  if (r->d_blocks.n < nb && (rc = r->d_blocks.alloc(nb))) return rc;
+  RingArgs a = ring_args(r);
+  int maxn = 0;
+  for (int c = 0; c < r->nchain; ++c) maxn = std::max(maxn, r->n[c]);
+  dim3 grid((maxn * r->D + 127) / 128, r->nchain);
+  return dispatch_pes(r->pes, [&]<int K, int M>() -> int {
+    ring_hblock_kernel<K, M><<<grid, 128, 0, r->s>>>(a, r->d_x.p, singlewell, r->d_blocks.p);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  });
+}
+
+extern "C" int rp_ring_hessian_band(rp_ring* r, int chain, int singlewell, double* band) {
+  RP_REQUIRE(r && band && chain >= 0 && chain < r->nchain, "rp_ring_hessian_band: bad arguments");
+  int rc;
+  if ((rc = ring_blocks(r, singlewell))) return rc;
+  const int n = r->n[chain], D = r->D;
+  const int64_t tot = (int64_t)n * D, nband = (D + 1) * tot;
+  DevBuf<double> d_band;
+  if ((rc = d_band.alloc(nband))) return rc;
+  ring_band_kernel<<<(unsigned)((nband + 255) / 256), 256, 0, r->s>>>(
+      n, D, r->betan[chain] * r->betan[chain], r->d_massd.p,
+      r->d_blocks.p + (size_t)r->boff[chain] * D * D, d_band.p);
+  rc = hipGetLastError() == hipSuccess ? d_band.download(band, nband, r->s) : set_error(RP_ERR_HIP, "band kernel");
+  if (!rc) RP_HIP(hipStreamSynchronize(r->s));
+  d_band.free();
+  return rc;
+}
+
+namespace rp {
+// Eigenvalue of every chain's Hessian nearest zero: inverse iteration on the unshifted
+// factors already in d_sinv, then Rayleigh-quotient refinement (refactor H - eta I,
+// a few shifted inverse iterations), which converges even when eta_2 / eta_1 is near 1.
+// Leaves d_sinv holding shifted factors.
+int ring_lowest_eigen(rp_ring* r, double* eta) {
+  int rc;
+  if (r->d_work.n < 2 * (size_t)r->tot && (rc = r->d_work.alloc(2 * (size_t)r->tot))) return rc;
+  DevBuf<double> d_shift;
+  if ((rc = d_shift.alloc(r->nchain))) return rc;
+  std::vector<double> sh(r->nchain, 0.0);
+  ring_eta1_kernel<<<r->nchain, kRingThreads, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p,
+                                                         r->d_sinv.p, r->d_work.p, nullptr, 60, 1,
+                                                         r->d_eta.p);
+  RP_HIP(hipGetLastError());
+  for (int round = 0; round < 3; ++round) {
+    if ((rc = r->d_eta.download(sh.data(), r->nchain, r->s))) return rc;
+    RP_HIP(hipStreamSynchronize(r->s));
+    // shift just off the estimate so the shifted matrix stays factorable
+    for (double& v : sh) v = v * (1.0 - 1e-9);
+    if ((rc = d_shift.upload(sh.data(), r->nchain, r->s))) return rc;
+    ring_logdet_kernel<<<r->nchain, kRingThreads, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p,
+                                                             r->d_massd.p, r->d_blocks.p, d_shift.p,
+                                                             r->d_sinv.p, r->d_ld.p, r->d_nneg.p);
+    RP_HIP(hipGetLastError());
+    ring_eta1_kernel<<<r->nchain, kRingThreads, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p,
+                                                           r->d_sinv.p, r->d_work.p, d_shift.p, 4, 0,
+                                                           r->d_eta.p);
+    RP_HIP(hipGetLastError());
+  }
+  rc = r->d_eta.download(eta, r->nchain, r->s);
+  if (!rc) RP_HIP(hipStreamSynchronize(r->s));
+  d_shift.free();
+  return rc;
+}
+}  // namespace rp
+
+extern "C" int rp_ring_logdet(rp_ring* r, int singlewell, int skip_lowest, double* logdet_pos,
+                              int* nneg, double* eta1) {
+  RP_REQUIRE(r && logdet_pos, "rp_ring_logdet: bad arguments");
+  int rc;
+  const int D = r->D;
+  if ((rc = ring_blocks(r, singlewell))) return rc;
+  const size_t nb = (size_t)r->nbeads * D * D;
+  if (r->d_sinv.n < nb && (rc = r->d_sinv.alloc(nb))) return rc;
+  if ((rc = r->d_ld.alloc(r->nchain)) || (rc = r->d_nneg.alloc(r->nchain)) || (rc = r->d_eta.alloc(r->nchain)))
+    return rc;
+  ring_logdet_kernel<<<r->nchain, kRingThreads, 0, r->s>>>(D, r->d_n.p, r->d_boff.p, r->d_betan.p,
+                                                           r->d_massd.p, r->d_blocks.p, nullptr,
+                                                           r->d_sinv.p, r->d_ld.p, r->d_nneg.p);
+  RP_HIP(hipGetLastError());
+  std::vector<double> ld(r->nchain), et(r->nchain, 0.0);
+  std::vector<int> ng(r->nchain);
+  if ((rc = r->d_ld.download(ld.data(), r->nchain, r->s)) ||
+      (rc = r->d_nneg.download(ng.data(), r->nchain, r->s)))
+    return rc;
+  RP_HIP(hipStreamSynchronize(r->s));
+  if (skip_lowest || eta1) {
+    if ((rc = ring_lowest_eigen(r, et.data()))) return rc;
+  }
+  int status = RP_OK;
+  for (int c = 0; c < r->nchain; ++c) {
+    double v = ld[c];
+    if (skip_lowest) {
+      // sum over eta_i > 0, i >= 2 = log|det| - log|eta_1| - sum over the other eta < 0
+      if (ng[c] == 0 || (ng[c] == 1 && et[c] < 0.0)) {
+        v = ld[c] - std::log(std::fabs(et[c]));
+      } else {
+        status = set_error(RP_ERR_CONVERGE,
+                           "chain %d: %d negative eigenvalues; their sum is not fixed by det", c, ng[c]);
+        v = NAN;
+      }
+    } else if (ng[c] != 0) {
+      status = set_error(RP_ERR_CONVERGE, "chain %d: %d negative eigenvalues", c, ng[c]);
+      v = NAN;
+    }
+    logdet_pos[c] = v;
+    if (nneg) nneg[c] = ng[c];
+    if (eta1) eta1[c] = et[c];
+  }
+  return status;
+}
+
+extern "C" int rp_ring_param_grad(rp_ring* r, double* out, int* nparam) {
+  RP_REQUIRE(r && out, "rp_ring_param_grad: bad arguments");
+  if (r->pes->kind != PES_WATER)
+    return set_error(RP_ERR_ARG, "potential kind %d has no parameter sensitivities", r->pes->kind);
+  int rc;
+  DevBuf<double> d_out;
+  if ((rc = d_out.alloc(2 * (size_t)r->nchain))) return rc;
+  RingArgs a = ring_args(r);
+  switch (r->natom) {
+    case 3: ring_param_grad_kernel<1><<<r->nchain, kRingThreads, 0, r->s>>>(a, r->d_x.p, d_out.p); break;
+    case 6: ring_param_grad_kernel<2><<<r->nchain, kRingThreads, 0, r->s>>>(a, r->d_x.p, d_out.p); break;
+    default: ring_param_grad_kernel<6><<<r->nchain, kRingThreads, 0, r->s>>>(a, r->d_x.p, d_out.p); break;
+  }
+  if (hipGetLastError() != hipSuccess) rc = set_error(RP_ERR_HIP, "param grad kernel launch");
+  if (!rc) rc = d_out.download(out, 2 * (size_t)r->nchain, r->s);
+  if (!rc) RP_HIP(hipStreamSynchronize(r->s));
+  d_out.free();
+  if (nparam) *nparam = 2;
+  return rc;
+}

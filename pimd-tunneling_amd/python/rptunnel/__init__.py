@@ -10,5 +10,6 @@ from ._lib import RpError, init, lib
 from .pes import Pes
 from .verlet import Chain
 from .batch import Batch
+from .instanton import Ring
 
-__all__ = ["specs", "RpError", "init", "lib", "Pes", "Chain", "Batch"]
+__all__ = ["specs", "RpError", "init", "lib", "Pes", "Chain", "Batch", "Ring"]
