@@ -21,3 +21,10 @@ def rand_chain(spec, n, seed, scale=0.1):
     t = (np.arange(n) + 1.0) / (n + 1.0)
     x = np.outer(spec.well1, 1 - t) + np.outer(spec.well2, t)
     return x + scale * rng.standard_normal(x.shape)
+
+
+def spec_by_name(name):
+    from rptunnel import specs
+    return {"quartic": specs.quartic, "coupled2d": specs.coupled2d, "evb": specs.evb,
+            "water": lambda: specs.water_cluster(nmol=2),
+            "harmonic": lambda: specs.harmonic([1e-3, 2e-3], [0.1, -0.2], [1000.0, 2000.0])}[name]()
