@@ -49,7 +49,6 @@ def flops_per_bead_step(n, D):
 
 def build_polymers(sp, ns, ntraj, nlam):
     from rptunnel import specs
-    import oracle as O  # noqa: F401  (not used: node placement is driver arithmetic)
 
     xi, _ = gauleg(nlam)
     lampath, path = specs.linear_path(sp.well1, sp.well2, npath=2)
@@ -116,6 +115,21 @@ def cpu_sample(sp, ns, budget_s, threads):
     return out, detail
 
 
+def copy_probe(mib):
+    """One device-to-device copy of a known size: rocprofv3's FETCH_SIZE / WRITE_SIZE of
+    this dispatch against its 2 x mib MiB of algorithmic traffic calibrate the counters."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 8, "bytes_written": n * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,8 +139,18 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--steps-per-launch", type=int, default=8,
+                    help="steps fused into one kernel launch (warm-up and timed steps are "
+                         "multiples of it, so every launch of a family has the same shape)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick steps per launch by a timed trial instead")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (the HBM byte-counter "
+                         "calibration for the traffic figure) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): every trajectory "
               "is independent, so several GPUs would each run this same batch; see DESIGN.md "
@@ -145,12 +169,13 @@ def main():
     batch.set_path(lampath, path)
     batch.init_path()
     fams = batch.families()
-    if not args.no_autotune:
+    if args.autotune:
         best_steps, _ = batch.autotune(trial_steps=16)
         log(f"autotune: {best_steps} steps per launch")
+    else:
+        batch.set_launch(args.steps_per_launch)
     log(f"warm-up {args.warmup} steps")
     batch.run(args.warmup)
-    import ctypes
     rptunnel.lib().rp_finalize()  # device synchronize
     log(f"timing {args.steps} steps")
     t0 = time.perf_counter()

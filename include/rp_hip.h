@@ -184,9 +184,10 @@ int rp_ring_get_x(rp_ring *r, double *x);
 int rp_ring_um(rp_ring *r, double *energy, double *grad);
 /* Hessian-vector product of U_M (forward-over-reverse through the adjoint) */
 int rp_ring_hvp(rp_ring *r, const double *v, double *hv);
-/* UMhessian (:155-217) of chain `chain` in LAPACK lower band storage
- * band[(ndof+1) * n*ndof] (AB(1+i-j, j) at band[(i-j)*n*ndof + j]), bead blocks from
- * ndof HVP sweeps; singlewell reuses bead 1's block (:183-185) */
+/* UMhessian (:155-217) of chain `chain`: the reference's answer(ndof+1, n*ndof), LAPACK
+ * lower band storage in column-major order (AB(1+i-j, j) at band[j*(ndof+1) + i-j], 0-based),
+ * bead blocks from ndof forward-over-reverse sweeps; singlewell reuses bead 1's block
+ * (:183-185) */
 int rp_ring_hessian_band(rp_ring *r, int chain, int singlewell, double *band);
 /* log-determinant of every chain's mass-weighted ring Hessian by FP64 block LU, with
  * the reference's coupling pattern (:203).  logdet_pos[c] = sum of ln(eta) over eta > 0

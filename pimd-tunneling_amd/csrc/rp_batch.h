@@ -39,7 +39,6 @@ struct BatchArgs {
   int N, NPAD, D, ndim, natom, LD;
   int first;          // first polymer (batch order) of this launch's family
   int nsteps;
-  int init_path;      // 1: build the initial state (init_path) before stepping
   int64_t step0, imin;
   double dt, limit, stdev0;
   double c, sq, sp, c1sq, kappa;  // uniform rotation / O-step scalars (see above)

@@ -324,16 +324,73 @@ static int noise_end(rp_batch* b, const std::vector<int64_t>& saved) {
   return b->d_pi64.upload(pi.data(), pi.size(), b->s);
 }
 
-// One launch per family of `steps` steps (or the init_path pseudo-step), each bracketed
-// by its own events; per-family durations are accumulated into fam_ms.
+// init_path (verletmodule.f90:32-119) of every polymer of a family, one workgroup each:
+// NM momenta sqrt(m'/betan) z with the draws in (dim outer, atom inner) blocks of N
+// (:102-106), p = T v (:107-115), and the beads spread along the path spline,
+// x_k = splint(path, (k-1) xi/(N-1)) (:40-48).  It runs once per task, so a plain dense
+// product serves; the step kernel's launches then all have the same shape.
+__global__ void __launch_bounds__(256) batch_init_kernel(BatchArgs A) {
+  extern __shared__ double V[];
+  const int poly = A.first + blockIdx.x, tid = threadIdx.x;
+  const int N = A.N, NPAD = A.NPAD, D = A.D, ndim = A.ndim, natom = A.natom;
+  int64_t* pi = A.pi64 + (size_t)poly * kPI64;
+  double* st = A.state + pi[PI_STATE_OFF];
+  const int64_t noff = pi[PI_NOISE_OFF], cur = pi[PI_CURSOR], need = (int64_t)N * D;
+  const uint64_t strm = A.stream[poly];
+  if (A.noise && cur + need > A.noise_cap) {  // the same for every thread of the block
+    if (tid == 0) atomicOr(&A.flags[1], 1);
+    return;
+  }
+  for (int e = tid; e < N * D; e += blockDim.x) {
+    const int c = e / N, i = e - c * N;
+    const int atom = c / ndim, dim = c - atom * ndim;
+    const int64_t idx = cur + (int64_t)(dim * natom + atom) * N + i;
+    const double z = A.noise ? A.noise[noff + idx] : philox_normal(A.seed, strm, idx);
+    V[e] = (A.stdev0 * z) * (sqrt(A.massd[c]) * A.modes[i * kMODE + MD_LAM] * A.tau);
+  }
+  __syncthreads();
+  const double xi = A.pf64[(size_t)poly * kPF64 + PF_XI];
+  const double* lampath = A.pathd;
+  for (int e = tid; e < N * D; e += blockDim.x) {
+    const int c = e / N, i = e - c * N;
+    double acc = 0.0;
+    for (int k = 0; k < N; ++k) acc += A.T[(size_t)k * NPAD + i] * V[c * N + k];
+    st[(size_t)D * N + e] = acc;
+    st[e] = splint_dev(lampath, lampath + A.npath + (size_t)c * A.npath,
+                       lampath + A.npath * (1 + D) + (size_t)c * A.npath, A.npath,
+                       (double)i * xi / (double)(N - 1));
+  }
+  if (tid == 0) pi[PI_CURSOR] = cur + need;
+}
+
+static int run_init(rp_batch* b, const double* dn, int64_t cap) {
+  for (size_t fi = 0; fi < b->fam.size(); ++fi) {
+    Family& f = b->fam[fi];
+    BatchArgs a = base_args(b, f, 0.5 * b->dt);
+    a.nsteps = 0;
+    a.noise = dn;
+    a.noise_cap = cap;
+    const size_t lds = (size_t)f.N * b->D * sizeof(double);
+    RP_REQUIRE(lds <= 160 * 1024, "rp_batch_init_path: N*ndof = %d is too large",
+               f.N * b->D);
+    RP_HIP(hipFuncSetAttribute((const void*)batch_init_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    batch_init_kernel<<<f.count, 256, lds, b->s>>>(a);
+    RP_HIP(hipGetLastError());
+  }
+  RP_HIP(hipStreamSynchronize(b->s));
+  return RP_OK;
+}
+
+// One launch per family of `steps` steps, each bracketed by its own events; per-family
+// durations are accumulated into fam_ms.
 static int run_block(rp_batch* b, int steps, int64_t imin, const double* dn, int64_t cap,
-                     bool init, std::vector<double>& fam_ms) {
+                     std::vector<double>& fam_ms) {
   int rc;
   for (size_t fi = 0; fi < b->fam.size(); ++fi) {
     Family& f = b->fam[fi];
     BatchArgs a = base_args(b, f, 0.5 * b->dt);
-    a.nsteps = init ? 0 : steps;
-    a.init_path = init ? 1 : 0;
+    a.nsteps = steps;
     a.step0 = b->steps_done;
     a.imin = imin;
     a.noise = dn;
@@ -372,8 +429,7 @@ extern "C" int rp_batch_init_path(rp_batch* b, const double* noise) {
     }
     if ((rc = noise_begin(b, packed.data(), cap, saved))) return rc;
   }
-  std::vector<double> fam_ms;
-  if ((rc = run_block(b, 0, 0, noise ? b->d_noise.p : nullptr, cap, true, fam_ms))) return rc;
+  if ((rc = run_init(b, noise ? b->d_noise.p : nullptr, cap))) return rc;
   if (noise && (rc = noise_end(b, saved))) return rc;
   return batch_flags(b);
 }
@@ -436,7 +492,7 @@ extern "C" int rp_batch_run(rp_batch* b, int64_t nsteps, int64_t imin, const dou
   int64_t launches = 0, left = nsteps;
   while (left > 0) {
     const int steps = (int)std::min<int64_t>(left, b->steps_per_launch);
-    if ((rc = run_block(b, steps, imin, noise ? b->d_noise.p : nullptr, nnoise_per_poly, false,
+    if ((rc = run_block(b, steps, imin, noise ? b->d_noise.p : nullptr, nnoise_per_poly,
                         fam_ms)))
       return rc;
     ++launches;
@@ -508,7 +564,7 @@ extern "C" int rp_batch_autotune(rp_batch* b, int trial_steps, int* best_steps, 
     int64_t left = trial_steps;
     while (left > 0) {
       const int steps = (int)std::min<int64_t>(left, cs);
-      if ((rc = run_block(b, steps, INT64_MAX / 2, nullptr, 0, false, fam_ms))) return rc;
+      if ((rc = run_block(b, steps, INT64_MAX / 2, nullptr, 0, fam_ms))) return rc;
       left -= steps;
     }
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -7,6 +7,18 @@ namespace rp {
 
 constexpr int kBatchWaves = 4;
 
+// f(integral_constant<int, 0..N-1>) expanded at compile time: the accumulator indices
+// stay constants even where a loop body is too large for the unroller, so the
+// accumulators never fall back to scratch memory.
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 __device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -145,8 +157,8 @@ __device__ __forceinline__ void nm_phase(const BatchArgs& A, double* S, const do
   const int N = A.N, D = A.D, LD = A.LD;
   const int units = (A.NPAD >> 5) * DT;
   const double c = A.c, sq = A.sq, sp = A.sp;
-#pragma unroll
-  for (int u = 0; u < UPW; ++u) {
+  static_for<UPW>([&](auto U) __attribute__((always_inline)) {
+    constexpr int u = decltype(U)::value;
     const int unit = wave + u * W;
     if (unit < units) {
       const int rp = unit / DT, dtile = unit - rp * DT;
@@ -183,7 +195,7 @@ __device__ __forceinline__ void nm_phase(const BatchArgs& A, double* S, const do
         }
       }
     }
-  }
+  });
 }
 
 // B: p -= dt grad V(x) for every bead (verletmodule.f90:565-585)
@@ -380,15 +392,11 @@ __global__ void __launch_bounds__(kBatchWaves * 64)
   const int64_t need = (int64_t)N * D;  // draws per O-step / resample / re-initialisation
   __syncthreads();
   bool stop = false;
-  for (int s = A.init_path ? -1 : 0; s < A.nsteps; ++s) {
+  for (int s = 0; s < A.nsteps; ++s) {
     const int64_t ii = A.step0 + s + 1;  // the reference's step counter
     int ph[6], nph = 0;
     bool resample = false;
-    if (s < 0) {
-      // init_path before the first step (pimd_par.f90:329)
-      if (A.noise && cur + need > A.noise_cap) { stop = true; break; }
-      ph[nph++] = PH_REINIT;
-    } else if (langevin) {
+    if (langevin) {
       if (A.noise && cur + need > A.noise_cap) { stop = true; break; }
       ph[nph++] = PH_FWD_PILE;
       ph[nph++] = PH_BWD;
@@ -463,7 +471,7 @@ __global__ void __launch_bounds__(kBatchWaves * 64)
       }
       __syncthreads();
       // TI estimator after the step (verletmodule.f90:397-410, 236-244)
-      if (k == nph - 1 && !ti_done && s >= 0 && ii > A.imin) {
+      if (k == nph - 1 && !ti_done && ii > A.imin) {
         ti_done = true;
         if (tid == 0) {
           double contr = 0.0;

@@ -133,16 +133,17 @@ __global__ void ring_hblock_kernel(RingArgs A, const double* __restrict__ x, int
   for (int k = 0; k < D; ++k) blk[k] = gs[k].d;
 }
 
-// UMhessian's band (instantonmod.f90:155-217) from the blocks of one chain: LAPACK lower
-// band storage band[r*tot + col] = A(col + r, col), r = 0..D.  The inter-bead coupling
+// UMhessian's band (instantonmod.f90:155-217) from the blocks of one chain, in the
+// reference's array answer(ndof+1, n*ndof) (LAPACK lower band storage, column-major):
+// band[col*(D+1) + r] = A(col + r, col), r = 0..D.  The inter-bead coupling
 // -1/bn^2 is written only for beads i > 1 (1-based), as the reference does (:203).
 __global__ void ring_band_kernel(int n, int D, double bn2, const double* __restrict__ massd,
                                  const double* __restrict__ blocks, double* __restrict__ band) {
   const int64_t tot = (int64_t)n * D;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (D + 1) * tot) return;
-  const int r = (int)(t / tot);
-  const int64_t col = t - (int64_t)r * tot;
+  const int64_t col = t / (D + 1);
+  const int r = (int)(t - col * (D + 1));
   const int bead = (int)(col / D), i1 = (int)(col - (int64_t)bead * D);
   double v = 0.0;
   if (r == 0) {

@@ -90,11 +90,12 @@ class Ring:
         return self.unpack(hv)
 
     def UMhessian(self, chain=0, singlewell=False):
-        """LAPACK lower band storage (ndof+1, n*ndof), with the reference's coupling."""
+        """LAPACK lower band storage AB(ndof+1, n*ndof), with the reference's coupling;
+        the library fills it column-major, as the reference's answer(ndof+1, n*ndof)."""
         D, n = self.ndof, self.n[chain]
         band = np.zeros((D + 1) * n * D)
         L.check(L.lib().rp_ring_hessian_band(self.h, chain, int(singlewell), L.ptr(band)))
-        return band.reshape(D + 1, n * D)
+        return band.reshape(n * D, D + 1).T
 
     def logdet(self, singlewell=False, skip_lowest=True):
         """-> (sum of ln(eta) over the eigenvalues the reference keeps, nneg, eta_1)."""
