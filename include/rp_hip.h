@@ -161,8 +161,11 @@ int rp_batch_get_cursor(rp_batch *b, int64_t *cursor);
 /* families of equal (n, betan), one fused-kernel launch each: n[f], count[f] */
 int rp_batch_families(rp_batch *b, int *nfam, int *n, int *count);
 /* launch shape: steps per launch (>= 1; 0 keeps the current value) and waves per
- * polymer workgroup (4; 0 keeps).  rp_batch_autotune times the step-block candidates on
- * the live batch (state is saved and restored) and keeps the fastest. */
+ * polymer workgroup (4 or 8 for every family; 0 keeps each family's value; 8 applies to
+ * families of at most 16 MFMA units, the others keep 4).  rp_batch_autotune times, on the
+ * live batch (state saved and restored), 4 and 8 waves per family from each family's own
+ * launch timings, then the steps-per-launch candidates; it keeps the fastest and reports
+ * the steps and the majority wave count. */
 int rp_batch_set_launch(rp_batch *b, int steps_per_launch, int waves);
 int rp_batch_autotune(rp_batch *b, int trial_steps, int *best_steps, int *best_waves);
 /* hipEvent-timed average duration of one fused-kernel launch per family in the last

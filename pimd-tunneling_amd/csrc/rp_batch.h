@@ -57,6 +57,7 @@ struct BatchArgs {
   int64_t noise_cap;
   const int32_t* kicks;         // injected Poisson draws or nullptr
   int64_t kick_cap;
+  int waves;          // waves per workgroup of the step kernel (4 or 8)
   int npath;
   const double* pathd;          // [lampath (npath) | path (D*npath) | y2 (D*npath)]
   int* flags;                   // [0] NaN, [1] injected draws exhausted

@@ -19,6 +19,7 @@ struct Family {
   int N = 0, NPAD = 0;
   double betan = 0;
   int first = 0, count = 0;  // polymers [first, first+count) in batch order
+  int waves = 4;             // waves per workgroup of this family's step kernel (4 or 8)
   DevBuf<double> d_T, d_modes;
 };
 }  // namespace
@@ -94,10 +95,21 @@ static int batch_flags(rp_batch* b) {
   return RP_OK;
 }
 
+// Waves per workgroup by default: 8 (two per SIMD, which hides the latency of the
+// elementwise phases) when the MFMA units spread over 8 waves at least as evenly as over
+// 4 and there are at least 6 of them and at most 16; 4 otherwise.
+static int default_waves(int DT, int NPAD) {
+  const int units = (NPAD >> 5) * DT;
+  if (units < 6 || units > 16) return 4;
+  const double even8 = units / (8.0 * ((units + 7) / 8)), even4 = units / (4.0 * ((units + 3) / 4));
+  return even8 >= even4 ? 8 : 4;
+}
+
 static BatchArgs base_args(rp_batch* b, const Family& f, double h) {
   BatchArgs a;
   memset(&a, 0, sizeof(a));
   a.pes = b->pes->dev();
+  a.waves = f.waves;
   a.N = f.N;
   a.NPAD = f.NPAD;
   a.D = b->D;
@@ -201,6 +213,7 @@ extern "C" int rp_batch_create(rp_pes* pes, const double* mass, int npoly,
     f.betan = kv.first.second;
     f.first = (int)b->order.size();
     f.count = (int)kv.second.size();
+    f.waves = default_waves(b->DT, f.NPAD);
     for (int p : kv.second) b->order.push_back(p);
     b->fam.push_back(std::move(f));
   }
@@ -534,10 +547,18 @@ extern "C" int rp_batch_get_cursor(rp_batch* b, int64_t* cursor) {
   return RP_OK;
 }
 
+// 8 waves per workgroup needs at most 16 MFMA units (2 per wave)
+static bool eight_waves_fit(const rp_batch* b, const Family& f) {
+  return (f.NPAD >> 5) * b->DT <= 16;
+}
+
 extern "C" int rp_batch_set_launch(rp_batch* b, int steps_per_launch, int waves) {
   RP_REQUIRE(b && steps_per_launch >= 0, "rp_batch_set_launch: bad arguments");
-  RP_REQUIRE(waves == 0 || waves == 4, "rp_batch_set_launch: waves per workgroup must be 4");
+  RP_REQUIRE(waves == 0 || waves == 4 || waves == 8,
+             "rp_batch_set_launch: waves per workgroup must be 4 or 8");
   if (steps_per_launch > 0) b->steps_per_launch = steps_per_launch;
+  if (waves)
+    for (Family& f : b->fam) f.waves = (waves == 8 && eight_waves_fit(b, f)) ? 8 : 4;
   return RP_OK;
 }
 
@@ -552,35 +573,63 @@ extern "C" int rp_batch_autotune(rp_batch* b, int trial_steps, int* best_steps, 
       (rc = b->d_pi64.download(pi.data(), pi.size(), b->s)))
     return rc;
   RP_HIP(hipStreamSynchronize(b->s));
+  auto restore = [&]() -> int {
+    int r;
+    if ((r = b->d_state.upload(st.data(), ns, b->s)) || (r = b->d_pf64.upload(pf.data(), pf.size(), b->s)) ||
+        (r = b->d_pi64.upload(pi.data(), pi.size(), b->s)))
+      return r;
+    RP_HIP(hipStreamSynchronize(b->s));
+    return RP_OK;
+  };
+  auto trial = [&](int cs, std::vector<double>& fam_ms, double& sec) -> int {
+    fam_ms.assign(b->fam.size(), 0.0);
+    auto t0 = std::chrono::steady_clock::now();
+    int64_t left = trial_steps;
+    while (left > 0) {
+      const int steps = (int)std::min<int64_t>(left, cs);
+      int r;
+      if ((r = run_block(b, steps, INT64_MAX / 2, nullptr, 0, fam_ms))) return r;
+      left -= steps;
+    }
+    sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return restore();
+  };
   const int64_t steps0 = b->steps_done;
+  // 1. waves per family, from each family's own hipEvent-timed launches
+  const size_t nf = b->fam.size();
+  std::vector<double> best_ms(nf, 1e300), fam_ms;
+  std::vector<int> best_w(nf, 4);
+  double sec = 0.0;
+  for (int w : {4, 8}) {
+    for (Family& f : b->fam) f.waves = (w == 8 && eight_waves_fit(b, f)) ? 8 : 4;
+    if ((rc = trial(std::min(b->steps_per_launch, trial_steps), fam_ms, sec))) return rc;
+    for (size_t fi = 0; fi < nf; ++fi)
+      if (b->fam[fi].waves == w && fam_ms[fi] < best_ms[fi]) {
+        best_ms[fi] = fam_ms[fi];
+        best_w[fi] = w;
+      }
+  }
+  int n8 = 0;
+  for (size_t fi = 0; fi < nf; ++fi) {
+    b->fam[fi].waves = best_w[fi];
+    n8 += best_w[fi] == 8;
+  }
+  // 2. steps per launch
   const int cands[] = {8, 32, 128};
   double best = 1e300;
   int bs = b->steps_per_launch;
   for (int cs : cands) {
     if (cs > trial_steps) continue;
-    b->steps_per_launch = cs;
-    std::vector<double> fam_ms(b->fam.size(), 0.0);
-    auto t0 = std::chrono::steady_clock::now();
-    int64_t left = trial_steps;
-    while (left > 0) {
-      const int steps = (int)std::min<int64_t>(left, cs);
-      if ((rc = run_block(b, steps, INT64_MAX / 2, nullptr, 0, fam_ms))) return rc;
-      left -= steps;
-    }
-    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if ((rc = trial(cs, fam_ms, sec))) return rc;
     if (sec < best) {
       best = sec;
       bs = cs;
     }
-    if ((rc = b->d_state.upload(st.data(), ns, b->s)) || (rc = b->d_pf64.upload(pf.data(), pf.size(), b->s)) ||
-        (rc = b->d_pi64.upload(pi.data(), pi.size(), b->s)))
-      return rc;
-    RP_HIP(hipStreamSynchronize(b->s));
   }
   b->steps_done = steps0;
   b->steps_per_launch = bs;
   if (best_steps) *best_steps = bs;
-  if (best_waves) *best_waves = 4;
+  if (best_waves) *best_waves = 2 * n8 > (int)nf ? 8 : 4;  // the majority choice
   return batch_flags(b);
 }
 

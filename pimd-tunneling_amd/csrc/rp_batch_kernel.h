@@ -5,7 +5,6 @@
 
 namespace rp {
 
-constexpr int kBatchWaves = 4;
 
 // f(integral_constant<int, 0..N-1>) expanded at compile time: the accumulator indices
 // stay constants even where a loop body is too large for the unroller, so the
@@ -30,68 +29,67 @@ __device__ __forceinline__ double batch_draw(const BatchArgs& A, int64_t noff, u
 }
 
 // acc = T S for all units of this wave (see rp_batch.h for the unit layout).
-template <int DT, int UPW>
+template <int DT, int UPW, int W>
 __device__ __forceinline__ void gemm_T(const double* __restrict__ T, int NPAD, const double* S,
                                        int LD, int wave, int m, int g, d4 (&acc)[UPW][4]) {
-  constexpr int W = kBatchWaves;
   const int units = (NPAD >> 5) * DT;
 #pragma unroll
   for (int u = 0; u < UPW; ++u)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[u][c] = d4{0.0, 0.0, 0.0, 0.0};
-  const int nk = NPAD >> 2;
-  // Software pipeline: the fragments of K-chunk kc+1 are loaded while the MFMAs of
-  // chunk kc run (one wavefront per SIMD has no other wave to hide load latency).
+  const int nk = NPAD >> 2;  // a multiple of 8: NPAD is a multiple of 32
+  // Two register sets in ping-pong: the fragments of K-chunk kc+1 are loaded into one
+  // set while the MFMAs of chunk kc read the other, so a load has a whole chunk of
+  // MFMAs (16 x UPW x 4 cycles x 16) to arrive and no register copy sits between them.
   // wave is wave-uniform (an SGPR), so the unit loop has no divergent control flow; a
   // slot past the last unit recomputes the last unit and its result is dropped by the
-  // consumers, which keeps the accumulators in AGPRs through the loop.
-  int tcol[UPW], scol[UPW];
+  // consumers, which keeps every MFMA unconditional.
+  int toff[UPW], soff[UPW];
 #pragma unroll
   for (int u = 0; u < UPW; ++u) {
     const int unit = min(wave + u * W, units - 1);
     const int rp = unit / DT, dtile = unit - rp * DT;
-    tcol[u] = rp * 32 + m;
-    scol[u] = dtile * 16 + m;
+    toff[u] = g * NPAD + rp * 32 + m;
+    soff[u] = g * LD + dtile * 16 + m;
   }
-  double fa0[UPW], fa1[UPW], fbx[UPW], fbp[UPW];
-  auto load = [&](int kc) {
-    const int k = 4 * kc + g;
-    const double* Tk = T + (size_t)k * NPAD;
-    const double* Sk = S + k * LD;
+  double a0[2][UPW], a1[2][UPW], bx[2][UPW], bp[2][UPW];
+  auto load = [&](int set, int kc) __attribute__((always_inline)) {
+    const double* Tk = T + (size_t)(4 * kc) * NPAD;  // wave-uniform chunk base
+    const double* Sk = S + 4 * kc * LD;
 #pragma unroll
     for (int u = 0; u < UPW; ++u) {
-      fa0[u] = Tk[tcol[u]];
-      fa1[u] = Tk[tcol[u] + 16];
-      fbx[u] = Sk[scol[u]];
-      fbp[u] = Sk[DT * 16 + scol[u]];
+      a0[set][u] = Tk[toff[u]];
+      a1[set][u] = Tk[toff[u] + 16];
+      bx[set][u] = Sk[soff[u]];
+      bp[set][u] = Sk[soff[u] + DT * 16];
     }
   };
-  load(0);
-  for (int kc = 0; kc < nk; ++kc) {
-    double a0[UPW], a1[UPW], bx[UPW], bp[UPW];
+  auto mma = [&](int set) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < UPW; ++u) {
-      a0[u] = fa0[u];
-      a1[u] = fa1[u];
-      bx[u] = fbx[u];
-      bp[u] = fbp[u];
+      acc[u][0] = mfma_f64(a0[set][u], bx[set][u], acc[u][0]);
+      acc[u][1] = mfma_f64(a0[set][u], bp[set][u], acc[u][1]);
+      acc[u][2] = mfma_f64(a1[set][u], bx[set][u], acc[u][2]);
+      acc[u][3] = mfma_f64(a1[set][u], bp[set][u], acc[u][3]);
     }
-    if (kc + 1 < nk) load(kc + 1);
-#pragma unroll
-    for (int u = 0; u < UPW; ++u) {
-      acc[u][0] = mfma_f64(a0[u], bx[u], acc[u][0]);
-      acc[u][1] = mfma_f64(a0[u], bp[u], acc[u][1]);
-      acc[u][2] = mfma_f64(a1[u], bx[u], acc[u][2]);
-      acc[u][3] = mfma_f64(a1[u], bp[u], acc[u][3]);
-    }
+  };
+  load(0, 0);
+  for (int kc = 0; kc < nk; kc += 2) {
+    load(1, kc + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(0);
+    __builtin_amdgcn_sched_barrier(0);
+    load(0, kc + 2 < nk ? kc + 2 : kc + 1);  // the last prefetch re-reads a valid chunk
+    __builtin_amdgcn_sched_barrier(0);
+    mma(1);
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
 // S <- acc (x and/or p columns)
-template <int DT, int UPW>
+template <int DT, int UPW, int W>
 __device__ __forceinline__ void store_acc(double* S, int LD, int NPAD, int wave, int m, int g,
                                           const d4 (&acc)[UPW][4], bool store_x) {
-  constexpr int W = kBatchWaves;
   const int units = (NPAD >> 5) * DT;
 #pragma unroll
   for (int u = 0; u < UPW; ++u) {
@@ -148,12 +146,11 @@ __device__ __forceinline__ void noise4(const BatchArgs& A, int64_t noff, uint64_
 
 // Normal-mode phase on the forward-transform accumulators: q = acc_x - beadvec,
 // A(dt/2) [O A(dt/2)] per element, then S <- [q + beadvec | p~].
-template <int DT, int UPW>
+template <int DT, int UPW, int W>
 __device__ __forceinline__ void nm_phase(const BatchArgs& A, double* S, const double* MT,
                                          int wave, int m, int g, d4 (&acc)[UPW][4],
                                          const double* __restrict__ pa, int64_t noff,
                                          uint64_t strm, int64_t cur, bool with_o, bool& nan) {
-  constexpr int W = kBatchWaves;
   const int N = A.N, D = A.D, LD = A.LD;
   const int units = (A.NPAD >> 5) * DT;
   const double c = A.c, sq = A.sq, sp = A.sp;
@@ -199,10 +196,9 @@ __device__ __forceinline__ void nm_phase(const BatchArgs& A, double* S, const do
 }
 
 // B: p -= dt grad V(x) for every bead (verletmodule.f90:565-585)
-template <int KIND, int MAXD, int DT>
+template <int KIND, int MAXD, int DT, int W>
 __device__ __forceinline__ void pes_kick(const BatchArgs& A, double* S, const double* PL, int tid,
                                          bool& nan) {
-  constexpr int W = kBatchWaves;
   const int N = A.N, D = A.D, LD = A.LD;
   for (int b = tid; b < N; b += W * 64) {
     double xs[MAXD], gs[MAXD];
@@ -223,9 +219,12 @@ __device__ __forceinline__ void pes_kick(const BatchArgs& A, double* S, const do
   }
 }
 
-// Threads per bead for the EVB kick: every thread of the workgroup works when N < 256.
-__host__ __device__ inline int evb_kick_tpb(int N) {
-  return N <= 32 ? 8 : N <= 64 ? 4 : N <= 128 ? 2 : 1;
+// Threads per bead for the EVB kick (1, 2, 4 or 8): as many as the workgroup's
+// threads allow, so that every thread works when N < threads.
+__host__ __device__ inline int evb_kick_tpb(int N, int threads) {
+  int t = 8;
+  while (t > 1 && t * N > threads) t >>= 1;
+  return t;
 }
 
 // Sum over the TPB adjacent lanes of a group (TPB = 2, 4 or 8) with DPP lane moves:
@@ -321,13 +320,113 @@ __device__ __forceinline__ void evb_kick_split(const BatchArgs& A, double* S, co
   }
 }
 
+// B for the EVB on the matrix cores.  For a tile of 16 beads, both wells' coordinates
+// y = Wt x - cw are one GEMM (rows: 2 x 16*RT well rows; K: the dofs) and the mixed
+// gradient wa WtA^T hA + wb WtB^T hB is a second GEMM (rows: dofs; K: both wells' rows)
+// whose B operand is the first GEMM's accumulators, scaled in place: item j of row tile
+// rt holds row 16 rt + g + 4 j of bead m, exactly the operand of K-chunk 4 rt + j.
+// The per-bead well energies are reduced over the four lane groups with two xor
+// shuffles.  Tiles are dealt to waves round-robin; padded rows and columns are zero.
+template <int M, int DT, int W>
+__device__ __forceinline__ void evb_kick_mfma(const BatchArgs& A, double* S, const double* PL,
+                                              int wave, int m, int g, bool& nan) {
+  constexpr int RT = (M + 15) / 16;  // row tiles per well (== DT)
+  constexpr int KY = 4 * RT;         // K-chunks of the y GEMM (dofs, padded to 16 RT)
+  constexpr int KG = 8 * RT;         // K-chunks of the gradient GEMM (both wells' rows)
+  const int N = A.N, D = A.D, LD = A.LD;
+  constexpr int DC = DT * 16;
+  const double kappa = PL[0];
+  const double* WA = PL + 1;
+  const double* cA = WA + M * M;
+  const double* WB = cA + M;
+  const double* cB = WB + M * M;
+  const double* lam = cB + M;
+  const double* cub = lam + M;
+  const int ntile = A.NPAD >> 4;
+  for (int t = wave; t < ntile; t += W) {
+    const int b0 = t * 16;
+    d4 ay[2 * RT];
+#pragma unroll
+    for (int r = 0; r < 2 * RT; ++r) ay[r] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kc = 0; kc < KY; ++kc) {
+      const int j = 4 * kc + g;
+      const double bx = S[(b0 + m) * LD + j];
+#pragma unroll
+      for (int r = 0; r < 2 * RT; ++r) {
+        // unconditional load at a clamped index, then a select: no branch per operand
+        const int i = (r % RT) * 16 + m;
+        const double* Wt = r < RT ? WA : WB;
+        const double w = Wt[min(i, M - 1) * M + min(j, M - 1)];
+        ay[r] = mfma_f64((i < M && j < M) ? w : 0.0, bx, ay[r]);
+      }
+    }
+    // h = lam y + 3 c y^2 and the well energies, per bead m
+    double va = 0.0, vb = 0.0;
+#pragma unroll
+    for (int r = 0; r < 2 * RT; ++r)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const int i = (r % RT) * 16 + g + 4 * jj;
+        const bool ok = i < M;
+        const int ic = min(i, M - 1);
+        const double cwv = r < RT ? cA[ic] : cB[ic], lv = lam[ic], cv = cub[ic];
+        const double cw = ok ? cwv : 0.0, li = ok ? lv : 0.0, ci = ok ? cv : 0.0;
+        const double y = ay[r][jj] - cw, y2 = y * y;
+        const double v = (0.5 * li) * y2 + ci * y2 * y;
+        if (r < RT)
+          va += v;
+        else
+          vb += v;
+        ay[r][jj] = li * y + (3.0 * ci) * y2;
+      }
+    va += __shfl_xor(va, 16);
+    va += __shfl_xor(va, 32);
+    vb += __shfl_xor(vb, 16);
+    vb += __shfl_xor(vb, 32);
+    const double dl = va - vb, R = sqrt(0.25 * dl * dl + kappa * kappa), f = dl / (4.0 * R);
+    const double wa = 0.5 - f, wb = 0.5 + f;
+#pragma unroll
+    for (int r = 0; r < 2 * RT; ++r) ay[r] *= (r < RT ? wa : wb);
+    // gradient: G[dof][bead] = sum over both wells' rows i of Wt[i][dof] h_i
+    d4 ag[DT];
+#pragma unroll
+    for (int d = 0; d < DT; ++d) ag[d] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kc = 0; kc < KG; ++kc) {
+      const int r = kc / 4, jj = kc % 4;
+      const int i = (r % RT) * 16 + 4 * jj + g;
+      const double* Wt = r < RT ? WA : WB;
+      const double bh = ay[r][jj];
+#pragma unroll
+      for (int d = 0; d < DT; ++d) {
+        const int dof = d * 16 + m;
+        const double w = Wt[min(i, M - 1) * M + min(dof, M - 1)];
+        ag[d] = mfma_f64((i < M && dof < M) ? w : 0.0, bh, ag[d]);
+      }
+    }
+    const int bead = b0 + m;
+#pragma unroll
+    for (int d = 0; d < DT; ++d)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const int dof = d * 16 + g + 4 * jj;
+        if (bead < N && dof < D) {
+          const double np = S[bead * LD + DC + dof] - ag[d][jj] * A.dt;
+          nan |= (np != np);
+          S[bead * LD + DC + dof] = np;
+        }
+      }
+  }
+}
+
 // NM-space momenta of a resample / init_path (verletmodule.f90:102-109, 211-219):
 // v_i = sqrt(1/betan) z sqrt(beadmass_i), draws in (dim outer, atom inner) blocks of N.
-template <int DT>
+template <int DT, int W>
 __device__ __forceinline__ void write_resample(const BatchArgs& A, double* S, const double* MT,
                                                int tid, int64_t noff, uint64_t strm,
                                                int64_t cur) {
-  constexpr int W = kBatchWaves, DC = DT * 16;
+  constexpr int DC = DT * 16;
   const int N = A.N, NPAD = A.NPAD, D = A.D, LD = A.LD, natom = A.natom, ndim = A.ndim;
   for (int e = tid; e < NPAD * DC; e += W * 64) {
     const int c = e / NPAD, i = e - c * NPAD;
@@ -348,12 +447,12 @@ __device__ __forceinline__ void write_resample(const BatchArgs& A, double* S, co
 //   re-initialisation:          [resample] T [store p, path spline -> x]
 enum : int { PH_FWD_PILE = 0, PH_FWD_A = 1, PH_BWD = 2, PH_RESAMPLE = 3, PH_REINIT = 4 };
 
-template <int KIND, int MAXD, int DT, int UPW>
-__global__ void __launch_bounds__(kBatchWaves * 64)
+template <int KIND, int MAXD, int DT, int UPW, int W>
+__global__ void __launch_bounds__(W * 64)
     batch_kernel(BatchArgs A, int thermostat) {
   extern __shared__ double S[];
   __shared__ int s_flag;
-  constexpr int W = kBatchWaves, DC = DT * 16;
+  constexpr int DC = DT * 16;
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (SGPR)
   const int poly = A.first + blockIdx.x;
@@ -428,32 +527,24 @@ __global__ void __launch_bounds__(kBatchWaves * 64)
     for (int k = 0; k < nph; ++k) {
       const int p = ph[k];
       if (p == PH_FWD_PILE || (p == PH_FWD_A && nbwd == 1)) {
-        if constexpr (KIND == PES_EVB) {  // B(dt)
-          const int tpb = evb_kick_tpb(N);
-          if (tpb == 8)
-            evb_kick_split<MAXD, DT, 8>(A, S, PL, tid, nan);
-          else if (tpb == 4)
-            evb_kick_split<MAXD, DT, 4>(A, S, PL, tid, nan);
-          else if (tpb == 2)
-            evb_kick_split<MAXD, DT, 2>(A, S, PL, tid, nan);
-          else
-            pes_kick<KIND, MAXD, DT>(A, S, PL, tid, nan);
+        if constexpr (KIND == PES_EVB) {  // B(dt) on the matrix cores
+          evb_kick_mfma<MAXD, DT, W>(A, S, PL, wave, m, g, nan);
         } else {
-          pes_kick<KIND, MAXD, DT>(A, S, PL, tid, nan);
+          pes_kick<KIND, MAXD, DT, W>(A, S, PL, tid, nan);
         }
         __syncthreads();
       } else if (p == PH_RESAMPLE || p == PH_REINIT) {
-        write_resample<DT>(A, S, MT, tid, noff, strm, cur);
+        write_resample<DT, W>(A, S, MT, tid, noff, strm, cur);
         __syncthreads();
       }
       d4 acc[UPW][4];
-      gemm_T<DT, UPW>(A.T, NPAD, S, LD, wave, m, g, acc);
+      gemm_T<DT, UPW, W>(A.T, NPAD, S, LD, wave, m, g, acc);
       __syncthreads();
       if (p == PH_FWD_PILE || p == PH_FWD_A) {
-        nm_phase<DT, UPW>(A, S, MT, wave, m, g, acc, pa, noff, strm, cur, p == PH_FWD_PILE, nan);
+        nm_phase<DT, UPW, W>(A, S, MT, wave, m, g, acc, pa, noff, strm, cur, p == PH_FWD_PILE, nan);
         if (p == PH_FWD_PILE) cur += need;
       } else {
-        store_acc<DT, UPW>(S, LD, NPAD, wave, m, g, acc, p == PH_BWD);
+        store_acc<DT, UPW, W>(S, LD, NPAD, wave, m, g, acc, p == PH_BWD);
         if (p == PH_BWD) ++nbwd;
         if (p == PH_REINIT) {
           // init_path positions: x_k = splint(path, (k-1) xi/(n-1)) (verletmodule.f90:40-48)
@@ -532,21 +623,26 @@ __global__ void __launch_bounds__(kBatchWaves * 64)
 
 template <int KIND, int MAXD, int DT>
 int launch_batch_kind(const BatchArgs& a, int thermostat, int npoly, hipStream_t s) {
-  constexpr int W = kBatchWaves;
   const int units = (a.NPAD >> 5) * DT;
-  const int upw = (units + W - 1) / W;
   const size_t lds = batch_lds_bytes(a.NPAD, DT, KIND == PES_EVB ? evb_dev_size<MAXD>() : 0);
-  auto go = [&](auto kern) -> int {
+  auto go = [&](auto kern, int waves) -> int {
     RP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
-    kern<<<npoly, W * 64, lds, s>>>(a, thermostat);
+    kern<<<npoly, waves * 64, lds, s>>>(a, thermostat);
     RP_HIP(hipGetLastError());
     return RP_OK;
   };
-  if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1>);
-  if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2>);
-  if (upw <= 3) return go(batch_kernel<KIND, MAXD, DT, 3>);
-  if (upw <= 4) return go(batch_kernel<KIND, MAXD, DT, 4>);
+  if (a.waves == 8) {  // two waves per SIMD: units spread over 8 waves, at most 2 each
+    const int upw = (units + 7) / 8;
+    if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1, 8>, 8);
+    if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2, 8>, 8);
+    return set_error(RP_ERR_ARG, "batch: %d MFMA units are too many for 8 waves", units);
+  }
+  const int upw = (units + 3) / 4;
+  if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1, 4>, 4);
+  if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2, 4>, 4);
+  if (upw <= 3) return go(batch_kernel<KIND, MAXD, DT, 3, 4>, 4);
+  if (upw <= 4) return go(batch_kernel<KIND, MAXD, DT, 4, 4>, 4);
   return set_error(RP_ERR_ARG, "batch: %d MFMA units per wave is more than supported", upw);
 }
 
