@@ -108,39 +108,47 @@ __device__ __forceinline__ void store_acc(double* S, int LD, int NPAD, int wave,
   }
 }
 
-// Four normals for lane group g at modes i = ibase + g + 4j (j = 0..3) of one dof,
-// draw index base + i.  When N is even the draws of modes (i, i+1) share one Philox
-// block, so lane groups (2k, 2k+1) split the four blocks of their pairs and swap
-// halves with the partner lane 16 away: half the Philox/Box-Muller work.
-__device__ __forceinline__ void noise4(const BatchArgs& A, int64_t noff, uint64_t strm,
+// The normals of one unit's two row halves: z[h][j] for lane group g at modes
+// i = ibase + 16 h + g + 4 j of one dof, draw index base + i.  When N is even the draws of
+// modes (i, i+1) share one Philox block, so lane groups (2k, 2k+1) split the blocks of
+// their pairs and swap halves with the partner lane 16 away: half the Philox/Box-Muller
+// work.  The four blocks of a lane are independent chains, made side by side.
+__device__ __forceinline__ void noise8(const BatchArgs& A, int64_t noff, uint64_t strm,
                                        int64_t base, int ibase, int g, bool real_dof,
-                                       double (&z)[4]) {
+                                       double (&z)[2][4]) {
   const int N = A.N;
   if (A.noise || (N & 1)) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int i = ibase + g + 4 * j;
-      z[j] = (real_dof && i < N) ? batch_draw(A, noff, strm, base + i) : 0.0;
-    }
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = ibase + 16 * h + g + 4 * j;
+        z[h][j] = (real_dof && i < N) ? batch_draw(A, noff, strm, base + i) : 0.0;
+      }
     return;
   }
   // base is even (the cursor and dof*N are multiples of N): pair (i even, i+1) = one block
   const bool odd = g & 1;
-  const int i0 = ibase + (g & ~1);
-  double mine[2], other[2];
+  double mine[2][2], other[2][2];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int ie = i0 + 4 * (odd ? 2 + t : t);  // even lane: j = 0,1; odd lane: j = 2,3
-    double z0 = 0.0, z1 = 0.0;
-    if (real_dof && ie < N) philox_normal2(A.seed, strm, (uint64_t)(base + ie) >> 1, z0, z1);
-    mine[t] = odd ? z1 : z0;
-    other[t] = odd ? z0 : z1;
-  }
-  const double r0 = __shfl_xor(other[0], 16), r1 = __shfl_xor(other[1], 16);
-  if (!odd) {
-    z[0] = mine[0]; z[1] = mine[1]; z[2] = r0; z[3] = r1;
-  } else {
-    z[0] = r0; z[1] = r1; z[2] = mine[0]; z[3] = mine[1];
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      // even lane: j = 0, 1; odd lane: j = 2, 3
+      const int ie = ibase + 16 * h + (g & ~1) + 4 * (odd ? 2 + t : t);
+      double z0 = 0.0, z1 = 0.0;
+      if (real_dof && ie < N) philox_normal2(A.seed, strm, (uint64_t)(base + ie) >> 1, z0, z1);
+      mine[h][t] = odd ? z1 : z0;
+      other[h][t] = odd ? z0 : z1;
+    }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const double r0 = __shfl_xor(other[h][0], 16), r1 = __shfl_xor(other[h][1], 16);
+    if (!odd) {
+      z[h][0] = mine[h][0]; z[h][1] = mine[h][1]; z[h][2] = r0; z[h][3] = r1;
+    } else {
+      z[h][0] = r0; z[h][1] = r1; z[h][2] = mine[h][0]; z[h][3] = mine[h][1];
+    }
   }
 }
 
@@ -163,10 +171,11 @@ __device__ __forceinline__ void nm_phase(const BatchArgs& A, double* S, const do
       const bool dreal = dof < D;
       const double mc = A.massd[dof], imc = 1.0 / mc, sqm = sqrt(mc);
       const double av = pa[dof], bvv = pa[DT * 16 + dof];
+      double zz[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+      if (with_o) noise8(A, noff, strm, cur + (int64_t)dof * N, rp * 32, g, dreal, zz);
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        double z[4] = {0.0, 0.0, 0.0, 0.0};
-        if (with_o) noise4(A, noff, strm, cur + (int64_t)dof * N, rp * 32 + h * 16, g, dreal, z);
+        const double* z = zz[h];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int i = rp * 32 + h * 16 + g + 4 * j;

@@ -43,22 +43,20 @@ __host__ __device__ __forceinline__ void philox_uniform2(uint64_t seed, uint64_t
   u1 = ((double)(b >> 11) + 0.5) * 0x1.0p-53;
 }
 
-// Both normals of one block (draws 2*block and 2*block+1).
-__host__ __device__ __forceinline__ void philox_normal2(uint64_t seed, uint64_t stream,
-                                                        uint64_t block, double& z0, double& z1) {
+// Both normals of one block (draws 2*block and 2*block+1).  The angle 2 pi u1 goes
+// through sincospi(2 u1), whose argument reduction is exact.
+__device__ __forceinline__ void philox_normal2(uint64_t seed, uint64_t stream, uint64_t block,
+                                               double& z0, double& z1) {
   double u0, u1;
   philox_uniform2(seed, stream, block, u0, u1);
   double r = sqrt(-2.0 * log(u0));
   double s, c;
-  double ang = 6.283185307179586 * u1;
-  s = sin(ang);
-  c = cos(ang);
+  sincospi(2.0 * u1, &s, &c);
   z0 = r * c;
   z1 = r * s;
 }
 
-__host__ __device__ __forceinline__ double philox_normal(uint64_t seed, uint64_t stream,
-                                                         int64_t index) {
+__device__ __forceinline__ double philox_normal(uint64_t seed, uint64_t stream, int64_t index) {
   double z0, z1;
   philox_normal2(seed, stream, (uint64_t)index >> 1, z0, z1);
   return (index & 1) ? z1 : z0;
