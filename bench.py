@@ -130,6 +130,57 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 8, "bytes_written": n * 8}))
 
 
+def instanton_sample(pes, sp, nchain=256, reps=10, maxiter=300, seed=4):
+    """The batched instanton workload on the same EVB surface: nchain open chains with N
+    uniform in [64, 1024] and beta in [1000, 8000] (RPI betan = beta/N, rpi_ser.f90:45).
+    Reports bead-evaluations per second of U_M with its adjoint gradient and of the HVP,
+    log-determinant bead blocks per second (lndetJ0 at well 1), and one batched L-BFGS
+    run from straight lines; each figure includes the host copies of its inputs/outputs."""
+    from rptunnel import Ring
+
+    rng = np.random.default_rng(seed)
+    ns = [int(v) for v in rng.integers(64, 1025, nchain)]
+    betas = rng.uniform(1000.0, 8000.0, nchain)
+    betan = [b / n for b, n in zip(betas, ns)]
+    ring = Ring(pes, sp.mass, ns, betan, sp.well1, sp.well2)
+    chains = []
+    for n in ns:
+        k = np.arange(1, n + 1)
+        chains.append((np.outer(sp.well1, n - k) + np.outer(sp.well2, k - 1)) / (n - 1))
+    ring.set_x(chains)
+    beads = sum(ns)
+    ring.UMforceenergy()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ring.UMforceenergy()
+    t_um = (time.perf_counter() - t0) / reps
+    v = ring.unpack(rng.standard_normal(ring.off[-1]))
+    ring.hvp(v)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ring.hvp(v)
+    t_hvp = (time.perf_counter() - t0) / reps
+    ring.set_x([np.repeat(sp.well1[:, None], n, axis=1) for n in ns])
+    ring.logdet(singlewell=True, skip_lowest=False)
+    t0 = time.perf_counter()
+    for _ in range(max(1, reps // 5)):
+        ring.logdet(singlewell=True, skip_lowest=False)
+    t_ld = (time.perf_counter() - t0) / max(1, reps // 5)
+    t0 = time.perf_counter()
+    _, act, it, st = ring.instanton(chains, pgtol=1e-5, factr=1e6, m=8, maxiter=maxiter)
+    t_opt = time.perf_counter() - t0
+    ring.close()
+    return {"workload": f"evb27 instantons: {nchain} chains, N in [64, 1024] (total {beads} beads), "
+                        "beta in [1000, 8000]",
+            "um_grad_bead_evals_per_s": beads / t_um,
+            "hvp_bead_evals_per_s": beads / t_hvp,
+            "logdet_bead_blocks_per_s": beads / t_ld,
+            "lbfgs": {"seconds": t_opt, "max_iterations": int(np.max(it)),
+                      "mean_iterations": float(np.mean(it)),
+                      "converged": int(np.sum(st == 0)), "chains": nchain,
+                      "pgtol": 1e-5, "factr": 1e6, "iteration_cap": maxiter}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +190,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-instanton", action="store_true",
+                    help="skip the batched-instanton sample reported beside the headline")
     ap.add_argument("--steps-per-launch", type=int, default=8,
                     help="steps fused into one kernel launch (warm-up and timed steps are "
                          "multiples of it, so every launch of a family has the same shape)")
@@ -212,6 +265,11 @@ def main():
     polys_per_n = args.ntraj * NLAMBDA
     plain_value = sum(n * polys_per_n for n in NS) / sum(tp[n] * polys_per_n for n in NS)
 
+    inst = None
+    if not args.no_instanton:
+        log("instanton sample (256 chains)")
+        inst = instanton_sample(pes, sp)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -250,6 +308,7 @@ def main():
                      "flops_per_bead_step": flops_per_bead_step(dn, D),
                      "families": fam_rows},
         "cpu_baseline": cpu,
+        "instanton": inst,
     }
     print(json.dumps(out), flush=True)
 
