@@ -195,49 +195,42 @@ __global__ void __launch_bounds__(kRingThreads) ring_param_grad_kernel(RingArgs 
 namespace {
 constexpr int kMaxD = 56;  // ndof bound of the log-determinant workspace (LDS)
 
-// LDL^T of the D x D symmetric matrix in Sm (LDS, row stride D) and its inverse into Si;
-// returns log|det| and the count of negative pivots through *lg / *neg (thread 0 valid).
-__device__ void ldlt_inverse(int D, double* Sm, double* Si, double* dg, double& lg, int& neg) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  // right-looking LDL^T, L stored strictly below the diagonal of Sm
-  for (int j = 0; j < D; ++j) {
-    const double dj = Sm[j * D + j];
-    __syncthreads();
-    for (int r = j + 1 + tid; r < D; r += nt) Sm[r * D + j] = Sm[r * D + j] / dj;
-    __syncthreads();
-    const int m = D - j - 1;
-    for (int e = tid; e < m * m; e += nt) {
-      const int r = j + 1 + e / m, cc = j + 1 + e % m;
-      if (cc <= r) Sm[r * D + cc] -= Sm[r * D + j] * dj * Sm[cc * D + j];
+constexpr int kPer = (kMaxD * kMaxD + kRingThreads - 1) / kRingThreads;  // elements per thread
+
+// In-place Gauss-Jordan inverse of the D x D matrix in M (LDS, row stride D), without
+// pivoting, every element updated in parallel at each of the D eliminations.  The pivot of
+// elimination k is the k-th pivot of the matrix's LDL^T factorisation (a ratio of leading
+// principal minors), so their logs sum to log|det| and their signs give the inertia
+// (Sylvester).  lg / neg are valid in thread 0.
+__device__ void gj_inverse(int D, double* M, double& lg, int& neg) {
+  const int tid = threadIdx.x, nt = blockDim.x, DD = D * D;
+  double l = 0.0;
+  int ng = 0;
+  for (int k = 0; k < D; ++k) {
+    const double p = M[k * D + k], ip = 1.0 / p;
+    double nv[kPer];
+    int cnt = 0;
+    for (int e = tid; e < DD; e += nt, ++cnt) {
+      const int r = e / D, cc = e - r * D;
+      const double mrk = M[r * D + k], mkc = M[k * D + cc];
+      double v;
+      if (r == k)
+        v = (cc == k) ? ip : mkc * ip;
+      else if (cc == k)
+        v = -mrk * ip;
+      else
+        v = M[e] - mrk * mkc * ip;
+      nv[cnt] = v;
     }
-    if (tid == 0) dg[j] = dj;
+    l += log(fabs(p));
+    ng += p < 0.0;
+    __syncthreads();
+    cnt = 0;
+    for (int e = tid; e < DD; e += nt, ++cnt) M[e] = nv[cnt];
     __syncthreads();
   }
-  if (tid == 0) {
-    double l = 0.0;
-    int ng = 0;
-    for (int j = 0; j < D; ++j) {
-      l += log(fabs(dg[j]));
-      ng += dg[j] < 0.0;
-    }
-    lg = l;
-    neg = ng;
-  }
-  // inverse: solve L Y = I, Y /= d, L^T X = Y; column e handled by thread e
-  for (int e = tid; e < D; e += nt) {
-    for (int r = 0; r < D; ++r) {
-      double s = (r == e) ? 1.0 : 0.0;
-      for (int k = 0; k < r; ++k) s -= Sm[r * D + k] * Si[k * D + e];
-      Si[r * D + e] = s;
-    }
-    for (int r = 0; r < D; ++r) Si[r * D + e] /= dg[r];
-    for (int r = D - 1; r >= 0; --r) {
-      double s = Si[r * D + e];
-      for (int k = r + 1; k < D; ++k) s -= Sm[k * D + r] * Si[k * D + e];
-      Si[r * D + e] = s;
-    }
-  }
-  __syncthreads();
+  lg = l;
+  neg = ng;
 }
 
 __global__ void __launch_bounds__(kRingThreads) ring_logdet_kernel(
@@ -245,7 +238,7 @@ __global__ void __launch_bounds__(kRingThreads) ring_logdet_kernel(
     const double* __restrict__ betan, const double* __restrict__ massd,
     const double* __restrict__ blocks, const double* __restrict__ shift, double* __restrict__ sinv,
     double* __restrict__ logdet, int* __restrict__ nneg) {
-  __shared__ double Sm[kMaxD * kMaxD], Si[kMaxD * kMaxD], dg[kMaxD];
+  __shared__ double Sm[kMaxD * kMaxD];
   const int c = blockIdx.x, n = nn[c], tid = threadIdx.x;
   const double bn2 = betan[c] * betan[c], c2 = 1.0 / (bn2 * bn2);
   const double sig = shift ? shift[c] : 0.0;  // factor H - sig I
@@ -266,13 +259,13 @@ __global__ void __launch_bounds__(kRingThreads) ring_logdet_kernel(
     __syncthreads();
     double lg;
     int ng;
-    ldlt_inverse(D, Sm, Si, dg, lg, ng);
+    gj_inverse(D, Sm, lg, ng);
     if (tid == 0) {
       lg_tot += lg;
       neg_tot += ng;
     }
     double* out = sinv + (size_t)(boff[c] + i) * D * D;
-    for (int e = tid; e < D * D; e += blockDim.x) out[e] = Si[e];
+    for (int e = tid; e < D * D; e += blockDim.x) out[e] = Sm[e];
     __syncthreads();
   }
   if (tid == 0) {
@@ -281,36 +274,43 @@ __global__ void __launch_bounds__(kRingThreads) ring_logdet_kernel(
   }
 }
 
+// Lanes per row of the D x D mat-vecs below: a power of two, at most 64 (one wave), with
+// D rows x lanes within the workgroup.
+__device__ __forceinline__ int matvec_lanes(int D) {
+  int q = 64;
+  while (q > 1 && q * D > kRingThreads) q >>= 1;
+  return q;
+}
+
+// tmp[r] = sum_k A[r][k] u_k for a D x D block A (global, row-major) and u = u0 + c u1
+// (u1 may be null); row r is summed by Q adjacent lanes and reduced with xor shuffles.
+__device__ __forceinline__ void block_matvec(int D, const double* __restrict__ A, const double* u0,
+                                             const double* u1, double c, double* tmp) {
+  const int tid = threadIdx.x, Q = matvec_lanes(D);
+  const int r = tid / Q, q = tid - r * Q;
+  double s = 0.0;
+  if (r < D)
+    for (int k = q; k < D; k += Q) s += A[r * D + k] * (u1 ? u0[k] + c * u1[k] : u0[k]);
+  for (int o = Q >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (r < D && q == 0) tmp[r] = s;
+  __syncthreads();
+}
+
 // y = H^-1 v through the stored block factors (block forward / back substitution).
 __device__ void block_solve(int D, int n, double c, const double* __restrict__ sinv,
                             double* w /* global, n*D: in v, out y */, double* tmp /* LDS D */) {
   const int tid = threadIdx.x;
   // forward: w_i += c S_{i-1}^-1 w_{i-1} for coupled (i-1, i), i >= 2
   for (int i = 2; i < n; ++i) {
-    const double* Sp = sinv + (size_t)(i - 1) * D * D;
-    for (int r = tid; r < D; r += blockDim.x) {
-      double s = 0.0;
-      for (int k = 0; k < D; ++k) s += Sp[r * D + k] * w[(size_t)(i - 1) * D + k];
-      tmp[r] = s;
-    }
-    __syncthreads();
+    block_matvec(D, sinv + (size_t)(i - 1) * D * D, w + (size_t)(i - 1) * D, nullptr, 0.0, tmp);
     for (int r = tid; r < D; r += blockDim.x) w[(size_t)i * D + r] += c * tmp[r];
     __syncthreads();
   }
   // back: y_{n-1} = S^-1 w_{n-1}; y_i = S_i^-1 (w_i + c y_{i+1}) when (i, i+1) coupled
   for (int i = n - 1; i >= 0; --i) {
-    const double* Sc = sinv + (size_t)i * D * D;
     const bool coupled = (i + 1 < n) && (i >= 1);
-    for (int r = tid; r < D; r += blockDim.x) {
-      double s = 0.0;
-      for (int k = 0; k < D; ++k) {
-        double rhs = w[(size_t)i * D + k];
-        if (coupled) rhs += c * w[(size_t)(i + 1) * D + k];
-        s += Sc[r * D + k] * rhs;
-      }
-      tmp[r] = s;
-    }
-    __syncthreads();
+    block_matvec(D, sinv + (size_t)i * D * D, w + (size_t)i * D,
+                 coupled ? w + (size_t)(i + 1) * D : nullptr, c, tmp);
     for (int r = tid; r < D; r += blockDim.x) w[(size_t)i * D + r] = tmp[r];
     __syncthreads();
   }

@@ -97,13 +97,17 @@ class Ring:
         L.check(L.lib().rp_ring_hessian_band(self.h, chain, int(singlewell), L.ptr(band)))
         return band.reshape(n * D, D + 1).T
 
-    def logdet(self, singlewell=False, skip_lowest=True):
-        """-> (sum of ln(eta) over the eigenvalues the reference keeps, nneg, eta_1)."""
+    def logdet(self, singlewell=False, skip_lowest=True, want_eta=None):
+        """-> (sum of ln(eta) over the eigenvalues the reference keeps, nneg, eta_1).
+        eta_1 (the lowest eigenvalue, by inverse iteration) is computed when skip_lowest
+        needs it or want_eta asks for it; otherwise it is returned as NaN."""
+        if want_eta is None:
+            want_eta = skip_lowest
         ld = np.zeros(self.nchain)
         nn = np.zeros(self.nchain, dtype=np.int32)
-        eta = np.zeros(self.nchain)
+        eta = np.full(self.nchain, np.nan)
         L.check(L.lib().rp_ring_logdet(self.h, int(singlewell), int(skip_lowest), L.ptr(ld),
-                                       L.iptr(nn), L.ptr(eta)))
+                                       L.iptr(nn), L.ptr(eta) if want_eta else None))
         return ld, nn, eta
 
     def instanton(self, chains, pgtol=1e-5, factr=1e6, m=8, maxiter=0):
