@@ -456,9 +456,9 @@ __device__ __forceinline__ void write_resample(const BatchArgs& A, double* S, co
 //   re-initialisation:          [resample] T [store p, path spline -> x]
 enum : int { PH_FWD_PILE = 0, PH_FWD_A = 1, PH_BWD = 2, PH_RESAMPLE = 3, PH_REINIT = 4 };
 
-template <int KIND, int MAXD, int DT, int UPW, int W>
+template <int KIND, int MAXD, int DT, int UPW, int W, bool LANGEVIN>
 __global__ void __launch_bounds__(W * 64)
-    batch_kernel(BatchArgs A, int thermostat) {
+    batch_kernel(BatchArgs A) {
   extern __shared__ double S[];
   __shared__ int s_flag;
   constexpr int DC = DT * 16;
@@ -473,7 +473,8 @@ __global__ void __launch_bounds__(W * 64)
   const double* pa = A.pdata + (size_t)poly * 3 * DC;
   const uint64_t strm = A.stream[poly];
   const int64_t noff = pi[PI_NOISE_OFF];
-  const bool langevin = thermostat == RP_THERMO_LANGEVIN;
+  // the thermostat is a compile-time choice: each kernel carries one step sequence only
+  constexpr bool langevin = LANGEVIN;
   int64_t cur = pi[PI_CURSOR];
   bool nan = false;
   // state -> LDS (zero padding); per-mode table -> LDS
@@ -630,29 +631,35 @@ __global__ void __launch_bounds__(W * 64)
   }
 }
 
-template <int KIND, int MAXD, int DT>
-int launch_batch_kind(const BatchArgs& a, int thermostat, int npoly, hipStream_t s) {
+template <int KIND, int MAXD, int DT, bool LANG>
+int launch_batch_thermo(const BatchArgs& a, int npoly, hipStream_t s) {
   const int units = (a.NPAD >> 5) * DT;
   const size_t lds = batch_lds_bytes(a.NPAD, DT, KIND == PES_EVB ? evb_dev_size<MAXD>() : 0);
   auto go = [&](auto kern, int waves) -> int {
     RP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
-    kern<<<npoly, waves * 64, lds, s>>>(a, thermostat);
+    kern<<<npoly, waves * 64, lds, s>>>(a);
     RP_HIP(hipGetLastError());
     return RP_OK;
   };
   if (a.waves == 8) {  // two waves per SIMD: units spread over 8 waves, at most 2 each
     const int upw = (units + 7) / 8;
-    if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1, 8>, 8);
-    if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2, 8>, 8);
+    if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1, 8, LANG>, 8);
+    if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2, 8, LANG>, 8);
     return set_error(RP_ERR_ARG, "batch: %d MFMA units are too many for 8 waves", units);
   }
   const int upw = (units + 3) / 4;
-  if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1, 4>, 4);
-  if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2, 4>, 4);
-  if (upw <= 3) return go(batch_kernel<KIND, MAXD, DT, 3, 4>, 4);
-  if (upw <= 4) return go(batch_kernel<KIND, MAXD, DT, 4, 4>, 4);
+  if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1, 4, LANG>, 4);
+  if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2, 4, LANG>, 4);
+  if (upw <= 3) return go(batch_kernel<KIND, MAXD, DT, 3, 4, LANG>, 4);
+  if (upw <= 4) return go(batch_kernel<KIND, MAXD, DT, 4, 4, LANG>, 4);
   return set_error(RP_ERR_ARG, "batch: %d MFMA units per wave is more than supported", upw);
+}
+
+template <int KIND, int MAXD, int DT>
+int launch_batch_kind(const BatchArgs& a, int thermostat, int npoly, hipStream_t s) {
+  if (thermostat == RP_THERMO_LANGEVIN) return launch_batch_thermo<KIND, MAXD, DT, true>(a, npoly, s);
+  return launch_batch_thermo<KIND, MAXD, DT, false>(a, npoly, s);
 }
 
 }  // namespace rp
