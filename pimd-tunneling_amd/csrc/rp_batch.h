@@ -50,6 +50,8 @@ struct BatchArgs {
   double* pf64;                 // per polymer doubles
   const uint64_t* stream;       // per polymer Philox stream id
   const double* T;              // NPAD*NPAD, zero padded
+  const double* Tf;             // folded halves [odd|even][HPAD][HPAD] (even N), else null
+  int fold;                     // 1: transforms through the folded halves (N even)
   const double* modes;          // per mode [mu, 1/mu, u, w, lam]
   const double* pdata;          // per polymer [a | b | dbdl], DC each, zero padded
   const double* massd;          // per dof mass, DC, padded with 1

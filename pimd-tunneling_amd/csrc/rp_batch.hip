@@ -20,7 +20,7 @@ struct Family {
   double betan = 0;
   int first = 0, count = 0;  // polymers [first, first+count) in batch order
   int waves = 4;             // waves per workgroup of this family's step kernel (4 or 8)
-  DevBuf<double> d_T, d_modes;
+  DevBuf<double> d_T, d_Tf, d_modes;
 };
 }  // namespace
 
@@ -51,6 +51,29 @@ struct rp_batch {
   std::vector<double> fam_ms;   // per family: summed launch time of the last run
   int64_t last_launches = 0;
 };
+
+// Folded halves of T for even N (H = N/2, HPAD = H rounded up to 16): row r of the odd
+// half is mode 2r+1 and row r of the even half mode 2r+2 (1-based), both over beads
+// 1..H.  Because sin(i (N+1-l) pi/(N+1)) = (-1)^(i+1) sin(i l pi/(N+1)), T x splits into
+// odd modes = Todd (x_l + x_{N+1-l}) and even modes = Teven (x_l - x_{N+1-l}), half the
+// multiply-adds; the identity holds to 3.4e-13 relative at N = 256 with the reference's
+// truncated pi.
+// Stored four times, each HPAD x HPAD and zero-padded: [odd^T | even^T | odd | even],
+// the transposes for the forward transform and the halves themselves for the backward one,
+// so both read rows of 16 consecutive entries.
+static void family_folded(const Family& f, const std::vector<double>& T, std::vector<double>& Tf) {
+  const int N = f.N, NP = f.NPAD, H = N / 2, HP = (H + 15) / 16 * 16;
+  const size_t Q = (size_t)HP * HP;
+  Tf.assign(4 * Q, 0.0);
+  for (int r = 0; r < H; ++r)
+    for (int k = 0; k < H; ++k) {
+      const double to = T[(size_t)(2 * r) * NP + k], te = T[(size_t)(2 * r + 1) * NP + k];
+      Tf[(size_t)k * HP + r] = to;
+      Tf[Q + (size_t)k * HP + r] = te;
+      Tf[2 * Q + (size_t)r * HP + k] = to;
+      Tf[3 * Q + (size_t)r * HP + k] = te;
+    }
+}
 
 static void family_tables(const rp_batch* b, const Family& f, std::vector<double>& T,
                           std::vector<double>& modes) {
@@ -143,6 +166,8 @@ static BatchArgs base_args(rp_batch* b, const Family& f, double h) {
   a.pf64 = b->d_pf64.p;
   a.stream = b->d_stream.p;
   a.T = f.d_T.p;
+  a.Tf = f.d_Tf.p;
+  a.fold = (f.N % 2 == 0) && f.d_Tf.p != nullptr;
   a.modes = f.d_modes.p;
   a.pdata = b->d_pdata.p;
   a.massd = b->d_massd.p;
@@ -262,6 +287,11 @@ extern "C" int rp_batch_create(rp_pes* pes, const double* mass, int npoly,
           std::vector<double> T, modes;
           family_tables(b, f, T, modes);
           if ((r = f.d_T.alloc(T.size())) || (r = f.d_T.upload(T.data(), T.size(), b->s))) return r;
+          if (f.N % 2 == 0) {
+            std::vector<double> Tf;
+            family_folded(f, T, Tf);
+            if ((r = f.d_Tf.alloc(Tf.size())) || (r = f.d_Tf.upload(Tf.data(), Tf.size(), b->s))) return r;
+          }
           if ((r = f.d_modes.alloc(modes.size())) || (r = f.d_modes.upload(modes.data(), modes.size(), b->s)))
             return r;
         }
@@ -278,6 +308,7 @@ extern "C" int rp_batch_destroy(rp_batch* b) {
   if (b->s) (void)hipStreamSynchronize(b->s);
   for (Family& f : b->fam) {
     f.d_T.free();
+    f.d_Tf.free();
     f.d_modes.free();
   }
   for (DevBuf<double>* d : {&b->d_state, &b->d_pf64, &b->d_pdata, &b->d_massd, &b->d_noise, &b->d_pathd})

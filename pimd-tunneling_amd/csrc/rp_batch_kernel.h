@@ -86,6 +86,118 @@ __device__ __forceinline__ void gemm_T(const double* __restrict__ T, int NPAD, c
   }
 }
 
+// The folded transforms (even N, H = N/2, HP = H padded to 16).  A unit is a block of 16
+// rows r of both halves and one dof tile; acc[u][0..1] hold the x / p columns of the odd
+// half, acc[u][2..3] those of the even half.
+//   forward:  odd r  (mode 2r+1) = sum_k Todd[r][k]  (x_k + x_{N-1-k})      (S rows k)
+//             even r (mode 2r+2) = sum_k Teven[r][k] (x_k - x_{N-1-k})      (S rows N-1-k)
+//             after the pre-fold that leaves the sums in rows k < H, the differences in
+//             rows N-1-k;
+//   backward: P_l = sum_r Todd[r][l] y_{2r},  Q_l = sum_r Teven[r][l] y_{2r+1}   (S rows = modes)
+//             x_l = P_l + Q_l, x_{N-1-l} = P_l - Q_l                       (store_fold)
+// The loop is the ping-pong of gemm_T over HP/4 K-chunks: half of gemm_T's MFMAs.
+template <int DT, int UPW, int W, bool BWD>
+__device__ __forceinline__ void gemm_fold(const double* __restrict__ Tf, int N, const double* S,
+                                          int LD, int wave, int m, int g, d4 (&acc)[UPW][4]) {
+  const int HP = ((N >> 1) + 15) / 16 * 16;
+  const int units = (HP >> 4) * DT;
+  const size_t Q = (size_t)HP * HP;
+  const double* To = Tf + (BWD ? 2 * Q : 0);
+  const double* Te = To + Q;
+#pragma unroll
+  for (int u = 0; u < UPW; ++u)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[u][c] = d4{0.0, 0.0, 0.0, 0.0};
+  const int nk = HP >> 2;  // a multiple of 4
+  int toff[UPW], soff[UPW];
+#pragma unroll
+  for (int u = 0; u < UPW; ++u) {
+    const int unit = min(wave + u * W, units - 1);
+    const int rp = unit / DT, dtile = unit - rp * DT;
+    toff[u] = g * HP + rp * 16 + m;
+    soff[u] = dtile * 16 + m;
+  }
+  double ao[2][UPW], ae[2][UPW], xo[2][UPW], po[2][UPW], xe[2][UPW], pe[2][UPW];
+  auto load = [&](int set, int kc) __attribute__((always_inline)) {
+    const size_t tb = (size_t)(4 * kc) * HP;  // wave-uniform chunk base
+    const int k = 4 * kc + g;
+    const int ro = BWD ? 2 * k : k;                       // odd-half operand row
+    const int re = BWD ? 2 * k + 1 : max(N - 1 - k, 0);   // even-half operand row
+#pragma unroll
+    for (int u = 0; u < UPW; ++u) {
+      ao[set][u] = To[tb + toff[u]];
+      ae[set][u] = Te[tb + toff[u]];
+      xo[set][u] = S[ro * LD + soff[u]];
+      po[set][u] = S[ro * LD + DT * 16 + soff[u]];
+      xe[set][u] = S[re * LD + soff[u]];
+      pe[set][u] = S[re * LD + DT * 16 + soff[u]];
+    }
+  };
+  auto mma = [&](int set) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < UPW; ++u) {
+      acc[u][0] = mfma_f64(ao[set][u], xo[set][u], acc[u][0]);
+      acc[u][1] = mfma_f64(ao[set][u], po[set][u], acc[u][1]);
+      acc[u][2] = mfma_f64(ae[set][u], xe[set][u], acc[u][2]);
+      acc[u][3] = mfma_f64(ae[set][u], pe[set][u], acc[u][3]);
+    }
+  };
+  load(0, 0);
+  for (int kc = 0; kc < nk; kc += 2) {
+    load(1, kc + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(0);
+    __builtin_amdgcn_sched_barrier(0);
+    load(0, kc + 2 < nk ? kc + 2 : kc + 1);  // the last prefetch re-reads a valid chunk
+    __builtin_amdgcn_sched_barrier(0);
+    mma(1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Pre-fold of the forward transform: rows k < H <- x_k + x_{N-1-k}, rows N-1-k <- the
+// difference, for every x and p column (each pair owned by one thread).
+template <int DT, int W>
+__device__ __forceinline__ void prefold(double* S, int N, int LD, int tid) {
+  constexpr int C2 = 2 * DT * 16;
+  const int H = N >> 1;
+  for (int e = tid; e < H * C2; e += W * 64) {
+    const int l = e / C2, c = e - l * C2;
+    const double a = S[l * LD + c], b = S[(N - 1 - l) * LD + c];
+    S[l * LD + c] = a + b;
+    S[(N - 1 - l) * LD + c] = a - b;
+  }
+}
+
+// S <- the backward folded accumulators: x_l = P + Q, x_{N-1-l} = P - Q (x and/or p).
+template <int DT, int UPW, int W>
+__device__ __forceinline__ void store_fold(double* S, int LD, int N, int wave, int m, int g,
+                                           const d4 (&acc)[UPW][4], bool store_x) {
+  const int H = N >> 1, HP = (H + 15) / 16 * 16;
+  const int units = (HP >> 4) * DT;
+  constexpr int DC = DT * 16;
+#pragma unroll
+  for (int u = 0; u < UPW; ++u) {
+    const int unit = wave + u * W;
+    if (unit < units) {
+      const int rp = unit / DT, dtile = unit - rp * DT;
+      const int col = dtile * 16 + m;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int l = rp * 16 + g + 4 * j;
+        if (l < H) {
+          if (store_x) {
+            S[l * LD + col] = acc[u][0][j] + acc[u][2][j];
+            S[(N - 1 - l) * LD + col] = acc[u][0][j] - acc[u][2][j];
+          }
+          S[l * LD + DC + col] = acc[u][1][j] + acc[u][3][j];
+          S[(N - 1 - l) * LD + DC + col] = acc[u][1][j] - acc[u][3][j];
+        }
+      }
+    }
+  }
+}
+
 // S <- acc (x and/or p columns)
 template <int DT, int UPW, int W>
 __device__ __forceinline__ void store_acc(double* S, int LD, int NPAD, int wave, int m, int g,
@@ -109,12 +221,14 @@ __device__ __forceinline__ void store_acc(double* S, int LD, int NPAD, int wave,
 }
 
 // The normals of one unit's two row halves: z[h][j] for lane group g at modes
-// i = ibase + 16 h + g + 4 j of one dof, draw index base + i.  When N is even the draws of
+// i = 32 rp + 16 h + g + 4 j of one dof (folded units: i = 2 (16 rp + g + 4 j) + h), draw
+// index base + i.  When N is even the draws of
 // modes (i, i+1) share one Philox block, so lane groups (2k, 2k+1) split the blocks of
 // their pairs and swap halves with the partner lane 16 away: half the Philox/Box-Muller
 // work.  The four blocks of a lane are independent chains, made side by side.
+template <bool FOLD>
 __device__ __forceinline__ void noise8(const BatchArgs& A, int64_t noff, uint64_t strm,
-                                       int64_t base, int ibase, int g, bool real_dof,
+                                       int64_t base, int rp, int g, bool real_dof,
                                        double (&z)[2][4]) {
   const int N = A.N;
   if (A.noise || (N & 1)) {
@@ -122,12 +236,25 @@ __device__ __forceinline__ void noise8(const BatchArgs& A, int64_t noff, uint64_
     for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int i = ibase + 16 * h + g + 4 * j;
+        const int i = FOLD ? 2 * (rp * 16 + g + 4 * j) + h : rp * 32 + 16 * h + g + 4 * j;
         z[h][j] = (real_dof && i < N) ? batch_draw(A, noff, strm, base + i) : 0.0;
       }
     return;
   }
   // base is even (the cursor and dof*N are multiples of N): pair (i even, i+1) = one block
+  if constexpr (FOLD) {
+    // folded units hold modes 2r and 2r+1 in the same lane: one block gives both
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i0 = 2 * (rp * 16 + g + 4 * j);
+      double z0 = 0.0, z1 = 0.0;
+      if (real_dof && i0 < N) philox_normal2(A.seed, strm, (uint64_t)(base + i0) >> 1, z0, z1);
+      z[0][j] = z0;
+      z[1][j] = z1;
+    }
+    return;
+  }
+  const int ibase = rp * 32;
   const bool odd = g & 1;
   double mine[2][2], other[2][2];
 #pragma unroll
@@ -154,7 +281,7 @@ __device__ __forceinline__ void noise8(const BatchArgs& A, int64_t noff, uint64_
 
 // Normal-mode phase on the forward-transform accumulators: q = acc_x - beadvec,
 // A(dt/2) [O A(dt/2)] per element, then S <- [q + beadvec | p~].
-template <int DT, int UPW, int W>
+template <int DT, int UPW, int W, bool FOLD>
 __device__ __forceinline__ void nm_phase(const BatchArgs& A, double* S, const double* MT,
                                          int wave, int m, int g, d4 (&acc)[UPW][4],
                                          const double* __restrict__ pa, int64_t noff,
@@ -172,13 +299,13 @@ __device__ __forceinline__ void nm_phase(const BatchArgs& A, double* S, const do
       const double mc = A.massd[dof], imc = 1.0 / mc, sqm = sqrt(mc);
       const double av = pa[dof], bvv = pa[DT * 16 + dof];
       double zz[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-      if (with_o) noise8(A, noff, strm, cur + (int64_t)dof * N, rp * 32, g, dreal, zz);
+      if (with_o) noise8<FOLD>(A, noff, strm, cur + (int64_t)dof * N, rp, g, dreal, zz);
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const double* z = zz[h];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const int i = rp * 32 + h * 16 + g + 4 * j;
+          const int i = FOLD ? 2 * (rp * 16 + g + 4 * j) + h : rp * 32 + h * 16 + g + 4 * j;
           const double* md = MT + i * kMODE;
           const double wm = mc * md[MD_MU], iwm = imc * md[MD_IMU];
           const double bv = av * md[MD_U] + bvv * md[MD_W];
@@ -359,6 +486,9 @@ __device__ __forceinline__ void evb_kick_mfma(const BatchArgs& A, double* S, con
     for (int r = 0; r < 2 * RT; ++r) ay[r] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int kc = 0; kc < KY; ++kc) {
+      // keep at most two K-chunks of operands in flight: the kick's registers stay below the
+      // budget of two workgroups per CU
+      if (kc % 2 == 0) __builtin_amdgcn_sched_barrier(0);
       const int j = 4 * kc + g;
       const double bx = S[(b0 + m) * LD + j];
 #pragma unroll
@@ -403,6 +533,7 @@ __device__ __forceinline__ void evb_kick_mfma(const BatchArgs& A, double* S, con
     for (int d = 0; d < DT; ++d) ag[d] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int kc = 0; kc < KG; ++kc) {
+      if (kc % 4 == 0) __builtin_amdgcn_sched_barrier(0);
       const int r = kc / 4, jj = kc % 4;
       const int i = (r % RT) * 16 + 4 * jj + g;
       const double* Wt = r < RT ? WA : WB;
@@ -456,8 +587,10 @@ __device__ __forceinline__ void write_resample(const BatchArgs& A, double* S, co
 //   re-initialisation:          [resample] T [store p, path spline -> x]
 enum : int { PH_FWD_PILE = 0, PH_FWD_A = 1, PH_BWD = 2, PH_RESAMPLE = 3, PH_REINIT = 4 };
 
-template <int KIND, int MAXD, int DT, int UPW, int W, bool LANGEVIN>
-__global__ void __launch_bounds__(W * 64)
+// One-unit 4-wave kernels (small bead counts, whose workgroups need little LDS) are held to
+// two waves per SIMD so that two polymers share a CU.
+template <int KIND, int MAXD, int DT, int UPW, int W, bool LANGEVIN, bool FOLD>
+__global__ void __launch_bounds__(W * 64, (W == 4 && UPW == 1) ? 2 : 1)
     batch_kernel(BatchArgs A) {
   extern __shared__ double S[];
   __shared__ int s_flag;
@@ -548,13 +681,27 @@ __global__ void __launch_bounds__(W * 64)
         __syncthreads();
       }
       d4 acc[UPW][4];
-      gemm_T<DT, UPW, W>(A.T, NPAD, S, LD, wave, m, g, acc);
+      const bool fwd = p == PH_FWD_PILE || p == PH_FWD_A;
+      if constexpr (FOLD) {
+        if (fwd) {
+          prefold<DT, W>(S, N, LD, tid);
+          __syncthreads();
+          gemm_fold<DT, UPW, W, false>(A.Tf, N, S, LD, wave, m, g, acc);
+        } else {
+          gemm_fold<DT, UPW, W, true>(A.Tf, N, S, LD, wave, m, g, acc);
+        }
+      } else {
+        gemm_T<DT, UPW, W>(A.T, NPAD, S, LD, wave, m, g, acc);
+      }
       __syncthreads();
       if (p == PH_FWD_PILE || p == PH_FWD_A) {
-        nm_phase<DT, UPW, W>(A, S, MT, wave, m, g, acc, pa, noff, strm, cur, p == PH_FWD_PILE, nan);
+        nm_phase<DT, UPW, W, FOLD>(A, S, MT, wave, m, g, acc, pa, noff, strm, cur, p == PH_FWD_PILE, nan);
         if (p == PH_FWD_PILE) cur += need;
       } else {
-        store_acc<DT, UPW, W>(S, LD, NPAD, wave, m, g, acc, p == PH_BWD);
+        if constexpr (FOLD)
+          store_fold<DT, UPW, W>(S, LD, N, wave, m, g, acc, p == PH_BWD);
+        else
+          store_acc<DT, UPW, W>(S, LD, NPAD, wave, m, g, acc, p == PH_BWD);
         if (p == PH_BWD) ++nbwd;
         if (p == PH_REINIT) {
           // init_path positions: x_k = splint(path, (k-1) xi/(n-1)) (verletmodule.f90:40-48)
@@ -631,7 +778,7 @@ __global__ void __launch_bounds__(W * 64)
   }
 }
 
-template <int KIND, int MAXD, int DT, bool LANG>
+template <int KIND, int MAXD, int DT, bool LANG, bool FOLD>
 int launch_batch_thermo(const BatchArgs& a, int npoly, hipStream_t s) {
   const int units = (a.NPAD >> 5) * DT;
   const size_t lds = batch_lds_bytes(a.NPAD, DT, KIND == PES_EVB ? evb_dev_size<MAXD>() : 0);
@@ -644,22 +791,30 @@ int launch_batch_thermo(const BatchArgs& a, int npoly, hipStream_t s) {
   };
   if (a.waves == 8) {  // two waves per SIMD: units spread over 8 waves, at most 2 each
     const int upw = (units + 7) / 8;
-    if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1, 8, LANG>, 8);
-    if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2, 8, LANG>, 8);
+    if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1, 8, LANG, FOLD>, 8);
+    if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2, 8, LANG, FOLD>, 8);
     return set_error(RP_ERR_ARG, "batch: %d MFMA units are too many for 8 waves", units);
   }
   const int upw = (units + 3) / 4;
-  if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1, 4, LANG>, 4);
-  if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2, 4, LANG>, 4);
-  if (upw <= 3) return go(batch_kernel<KIND, MAXD, DT, 3, 4, LANG>, 4);
-  if (upw <= 4) return go(batch_kernel<KIND, MAXD, DT, 4, 4, LANG>, 4);
+  if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1, 4, LANG, FOLD>, 4);
+  if (upw <= 2) return go(batch_kernel<KIND, MAXD, DT, 2, 4, LANG, FOLD>, 4);
+  if (upw <= 3) return go(batch_kernel<KIND, MAXD, DT, 3, 4, LANG, FOLD>, 4);
+  if (upw <= 4) return go(batch_kernel<KIND, MAXD, DT, 4, 4, LANG, FOLD>, 4);
   return set_error(RP_ERR_ARG, "batch: %d MFMA units per wave is more than supported", upw);
 }
 
-template <int KIND, int MAXD, int DT>
+// ALLOW_FOLD = false keeps a potential on the full transforms (fewer kernel instances).
+template <int KIND, int MAXD, int DT, bool ALLOW_FOLD = true>
 int launch_batch_kind(const BatchArgs& a, int thermostat, int npoly, hipStream_t s) {
-  if (thermostat == RP_THERMO_LANGEVIN) return launch_batch_thermo<KIND, MAXD, DT, true>(a, npoly, s);
-  return launch_batch_thermo<KIND, MAXD, DT, false>(a, npoly, s);
+  const bool lang = thermostat == RP_THERMO_LANGEVIN;
+  if constexpr (ALLOW_FOLD) {
+    if (a.fold) {
+      if (lang) return launch_batch_thermo<KIND, MAXD, DT, true, true>(a, npoly, s);
+      return launch_batch_thermo<KIND, MAXD, DT, false, true>(a, npoly, s);
+    }
+  }
+  if (lang) return launch_batch_thermo<KIND, MAXD, DT, true, false>(a, npoly, s);
+  return launch_batch_thermo<KIND, MAXD, DT, false, false>(a, npoly, s);
 }
 
 }  // namespace rp
