@@ -117,40 +117,46 @@ __device__ __forceinline__ void gemm_fold(const double* __restrict__ Tf, int N, 
     toff[u] = g * HP + rp * 16 + m;
     soff[u] = dtile * 16 + m;
   }
-  double ao[2][UPW], ae[2][UPW], xo[2][UPW], po[2][UPW], xe[2][UPW], pe[2][UPW];
-  auto load = [&](int set, int kc) __attribute__((always_inline)) {
+  // T fragments (global, long latency) in ping-pong; S fragments (LDS) loaded just before
+  // their MFMAs, which keeps the operand registers at 8 per unit
+  double ao[2][UPW], ae[2][UPW];
+  auto loadA = [&](int set, int kc) __attribute__((always_inline)) {
     const size_t tb = (size_t)(4 * kc) * HP;  // wave-uniform chunk base
-    const int k = 4 * kc + g;
-    const int ro = BWD ? 2 * k : k;                       // odd-half operand row
-    const int re = BWD ? 2 * k + 1 : max(N - 1 - k, 0);   // even-half operand row
 #pragma unroll
     for (int u = 0; u < UPW; ++u) {
       ao[set][u] = To[tb + toff[u]];
       ae[set][u] = Te[tb + toff[u]];
-      xo[set][u] = S[ro * LD + soff[u]];
-      po[set][u] = S[ro * LD + DT * 16 + soff[u]];
-      xe[set][u] = S[re * LD + soff[u]];
-      pe[set][u] = S[re * LD + DT * 16 + soff[u]];
     }
   };
-  auto mma = [&](int set) __attribute__((always_inline)) {
+  auto mma = [&](int set, int kc) __attribute__((always_inline)) {
+    const int k = 4 * kc + g;
+    const int ro = BWD ? 2 * k : k;                       // odd-half operand row
+    const int re = BWD ? 2 * k + 1 : max(N - 1 - k, 0);   // even-half operand row
+    double xo[UPW], po[UPW], xe[UPW], pe[UPW];
 #pragma unroll
     for (int u = 0; u < UPW; ++u) {
-      acc[u][0] = mfma_f64(ao[set][u], xo[set][u], acc[u][0]);
-      acc[u][1] = mfma_f64(ao[set][u], po[set][u], acc[u][1]);
-      acc[u][2] = mfma_f64(ae[set][u], xe[set][u], acc[u][2]);
-      acc[u][3] = mfma_f64(ae[set][u], pe[set][u], acc[u][3]);
+      xo[u] = S[ro * LD + soff[u]];
+      po[u] = S[ro * LD + DT * 16 + soff[u]];
+      xe[u] = S[re * LD + soff[u]];
+      pe[u] = S[re * LD + DT * 16 + soff[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < UPW; ++u) {
+      acc[u][0] = mfma_f64(ao[set][u], xo[u], acc[u][0]);
+      acc[u][1] = mfma_f64(ao[set][u], po[u], acc[u][1]);
+      acc[u][2] = mfma_f64(ae[set][u], xe[u], acc[u][2]);
+      acc[u][3] = mfma_f64(ae[set][u], pe[u], acc[u][3]);
     }
   };
-  load(0, 0);
+  loadA(0, 0);
   for (int kc = 0; kc < nk; kc += 2) {
-    load(1, kc + 1);
+    loadA(1, kc + 1);
     __builtin_amdgcn_sched_barrier(0);
-    mma(0);
+    mma(0, kc);
     __builtin_amdgcn_sched_barrier(0);
-    load(0, kc + 2 < nk ? kc + 2 : kc + 1);  // the last prefetch re-reads a valid chunk
+    loadA(0, kc + 2 < nk ? kc + 2 : kc + 1);  // the last prefetch re-reads a valid chunk
     __builtin_amdgcn_sched_barrier(0);
-    mma(1);
+    mma(1, kc + 1);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
