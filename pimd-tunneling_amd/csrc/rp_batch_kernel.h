@@ -361,107 +361,6 @@ __device__ __forceinline__ void pes_kick(const BatchArgs& A, double* S, const do
   }
 }
 
-// Threads per bead for the EVB kick (1, 2, 4 or 8): as many as the workgroup's
-// threads allow, so that every thread works when N < threads.
-__host__ __device__ inline int evb_kick_tpb(int N, int threads) {
-  int t = 8;
-  while (t > 1 && t * N > threads) t >>= 1;
-  return t;
-}
-
-// Sum over the TPB adjacent lanes of a group (TPB = 2, 4 or 8) with DPP lane moves:
-// quad_perm xor 1, quad_perm xor 2, then row_half_mirror pairs the two quads of 8.
-__device__ __forceinline__ double dpp_f64(double v, int ctrl) {
-  const int lo = __double2loint(v), hi = __double2hiint(v);
-  int nlo, nhi;
-  switch (ctrl) {
-    case 0xB1:
-      nlo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, false);
-      nhi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, false);
-      break;
-    case 0x4E:
-      nlo = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, false);
-      nhi = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, false);
-      break;
-    default:
-      nlo = __builtin_amdgcn_mov_dpp(lo, 0x141, 0xF, 0xF, false);
-      nhi = __builtin_amdgcn_mov_dpp(hi, 0x141, 0xF, 0xF, false);
-      break;
-  }
-  return __hiloint2double(nhi, nlo);
-}
-
-template <int TPB>
-__device__ __forceinline__ double group_sum(double v) {
-  if constexpr (TPB >= 2) v += dpp_f64(v, 0xB1);
-  if constexpr (TPB >= 4) v += dpp_f64(v, 0x4E);
-  if constexpr (TPB >= 8) v += dpp_f64(v, 0x141);
-  return v;
-}
-
-// B for the EVB with TPB adjacent lanes per bead (bead b = tid / TPB, part r = tid % TPB).
-// Part r holds the coordinates j = r, r+TPB, ...: each y_i = W_i . x - c_i is a partial dot
-// product summed over the group, and each part accumulates the gradient of its own j.
-template <int M, int DT, int TPB>
-__device__ __forceinline__ void evb_kick_split(const BatchArgs& A, double* S, const double* PL,
-                                               int tid, bool& nan) {
-  constexpr int JS = (M + TPB - 1) / TPB;
-  const int N = A.N, D = A.D, LD = A.LD;
-  const int b = tid / TPB, r = tid - (tid / TPB) * TPB;
-  const bool active = b < N;
-  const double kappa = PL[0];
-  const double* WA = PL + 1;
-  const double* cA = WA + M * M;
-  const double* WB = cA + M;
-  const double* cB = WB + M * M;
-  const double* lam = cB + M;
-  const double* cub = lam + M;
-  double xs[JS], gA[JS], gB[JS];
-  int jc[JS];
-#pragma unroll
-  for (int k = 0; k < JS; ++k) {
-    const int j = r + k * TPB;
-    jc[k] = j < M ? j : M - 1;  // clamped index: xs is 0 there, so the term vanishes
-    xs[k] = (active && j < D) ? S[b * LD + j] : 0.0;
-    gA[k] = 0.0;
-    gB[k] = 0.0;
-  }
-  double va = 0.0, vb = 0.0;
-#pragma unroll 1
-  for (int i = 0; i < M; ++i) {
-    const double* Wa = WA + i * M;
-    const double* Wb = WB + i * M;
-    double pa = 0.0, pb = 0.0;
-#pragma unroll
-    for (int k = 0; k < JS; ++k) {
-      pa += Wa[jc[k]] * xs[k];
-      pb += Wb[jc[k]] * xs[k];
-    }
-    const double ya = group_sum<TPB>(pa) - cA[i], yb = group_sum<TPB>(pb) - cB[i];
-    const double li = lam[i], ci = cub[i];
-    const double ya2 = ya * ya, yb2 = yb * yb;
-    va += (0.5 * li) * ya2 + ci * ya2 * ya;
-    vb += (0.5 * li) * yb2 + ci * yb2 * yb;
-    const double ha = li * ya + (3.0 * ci) * ya2, hb = li * yb + (3.0 * ci) * yb2;
-#pragma unroll
-    for (int k = 0; k < JS; ++k) {
-      gA[k] += Wa[jc[k]] * ha;
-      gB[k] += Wb[jc[k]] * hb;
-    }
-  }
-  const double dl = va - vb, R = sqrt(0.25 * dl * dl + kappa * kappa), f = dl / (4.0 * R);
-  const double wa = 0.5 - f, wb = 0.5 + f;
-#pragma unroll
-  for (int k = 0; k < JS; ++k) {
-    const int j = r + k * TPB;
-    if (active && j < D) {
-      const double np = S[b * LD + DT * 16 + j] - (wa * gA[k] + wb * gB[k]) * A.dt;
-      nan |= (np != np);
-      S[b * LD + DT * 16 + j] = np;
-    }
-  }
-}
-
 // B for the EVB on the matrix cores.  For a tile of 16 beads, both wells' coordinates
 // y = Wt x - cw are one GEMM (rows: 2 x 16*RT well rows; K: the dofs) and the mixed
 // gradient wa WtA^T hA + wb WtB^T hB is a second GEMM (rows: dofs; K: both wells' rows)
@@ -604,7 +503,7 @@ __global__ void __launch_bounds__(W * 64, (W == 4 && UPW == 1) ? 2 : 1)
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (SGPR)
   const int poly = A.first + blockIdx.x;
-  const int N = A.N, NPAD = A.NPAD, D = A.D, LD = A.LD, ndim = A.ndim, natom = A.natom;
+  const int N = A.N, NPAD = A.NPAD, D = A.D, LD = A.LD;
   double* MT = S + NPAD * LD;
   double* PL = MT + NPAD * kMODE;  // EVB matrices (evb_eval_at layout), KIND == PES_EVB only
   int64_t* pi = A.pi64 + (size_t)poly * kPI64;
@@ -727,19 +626,19 @@ __global__ void __launch_bounds__(W * 64, (W == 4 && UPW == 1) ? 2 : 1)
       // TI estimator after the step (verletmodule.f90:397-410, 236-244)
       if (k == nph - 1 && !ti_done && ii > A.imin) {
         ti_done = true;
-        if (tid == 0) {
+        if (wave == 0) {
+          // sum over dofs of m (-x_N) db/dxi: one dof per lane, then an xor-shuffle reduction
+          // (the reference's dim-outer order differs only in rounding)
           double contr = 0.0;
-          for (int jd = 0; jd < ndim; ++jd)
-            for (int ka = 0; ka < natom; ++ka) {
-              const int d = ka * ndim + jd;
-              contr = contr + A.massd[d] * (-S[(N - 1) * LD + d]) * pa[2 * DC + d];
-            }
+          for (int d = lane; d < D; d += 64) contr += A.massd[d] * (-S[(N - 1) * LD + d]) * pa[2 * DC + d];
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) contr += __shfl_xor(contr, o);
           const bool ok = !langevin || fabs(contr) < A.limit || A.limit < 0.0;
           if (ok) {
             tisum = tisum + contr;
             ++ticnt;
           }
-          s_flag = ok ? 0 : 1;
+          if (lane == 0) s_flag = ok ? 0 : 1;
         }
         if (langevin && A.limit >= 0.0) {
           __syncthreads();
