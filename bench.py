@@ -40,11 +40,14 @@ def log(msg):
 
 
 def flops_per_bead_step(n, D):
-    """Algorithmic FP64 work of one bead-step: two dense N x N transforms of the 2D
-    state columns (8 N D), the elementwise kick / rotations / O-step / shifts (21 D), and
-    the EVB potential: two wells, each y = W^T (x - xw) and W h (2 x 2 D^2) plus ~13 D."""
+    """FP64 work the kernel executes per bead-step (no credit for padding).  Transforms:
+    for even N the folded DST-I (two N/2 x N/2 halves per transform, 4 N D for the two
+    transforms of the 2D state columns) plus its pre-fold and recombination (4 D); for odd
+    N the dense N x N transforms (8 N D).  Elementwise kick / rotations / O-step / shifts:
+    21 D.  EVB potential: two wells, each y = W^T (x - xw) and W h (2 x 2 D^2) plus ~13 D."""
     w_pes = 8 * D * D + 26 * D
-    return 8 * n * D + 21 * D + w_pes
+    transforms = 4 * n * D + 4 * D if n % 2 == 0 else 8 * n * D
+    return transforms + 21 * D + w_pes
 
 
 def build_polymers(sp, ns, ntraj, nlam):
