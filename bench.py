@@ -138,7 +138,9 @@ def instanton_sample(pes, sp, nchain=256, reps=10, maxiter=300, seed=4):
     uniform in [64, 1024] and beta in [1000, 8000] (RPI betan = beta/N, rpi_ser.f90:45).
     Reports bead-evaluations per second of U_M with its adjoint gradient and of the HVP,
     log-determinant bead blocks per second (lndetJ0 at well 1), and one batched L-BFGS
-    run from straight lines; each figure includes the host copies of its inputs/outputs."""
+    run from straight lines; each figure includes the host copies of its inputs/outputs
+    (the gradient and HVP figures move n*ndof doubles each way per call), except the
+    energy-only U_M figure, which returns one double per chain."""
     from rptunnel import Ring
 
     rng = np.random.default_rng(seed)
@@ -157,6 +159,11 @@ def instanton_sample(pes, sp, nchain=256, reps=10, maxiter=300, seed=4):
     for _ in range(reps):
         ring.UMforceenergy()
     t_um = (time.perf_counter() - t0) / reps
+    ring.UM()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ring.UM()
+    t_e = (time.perf_counter() - t0) / reps
     v = ring.unpack(rng.standard_normal(ring.off[-1]))
     ring.hvp(v)
     t0 = time.perf_counter()
@@ -176,6 +183,7 @@ def instanton_sample(pes, sp, nchain=256, reps=10, maxiter=300, seed=4):
     return {"workload": f"evb27 instantons: {nchain} chains, N in [64, 1024] (total {beads} beads), "
                         "beta in [1000, 8000]",
             "um_grad_bead_evals_per_s": beads / t_um,
+            "um_energy_only_bead_evals_per_s": beads / t_e,
             "hvp_bead_evals_per_s": beads / t_hvp,
             "logdet_bead_blocks_per_s": beads / t_ld,
             "lbfgs": {"seconds": t_opt, "max_iterations": int(np.max(it)),

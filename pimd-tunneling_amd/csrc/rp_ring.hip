@@ -40,25 +40,49 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return r;
 }
 
+constexpr int kRingTile = 128;  // beads per workgroup of the per-bead ring kernels
+
+// The potential as seen by one workgroup: the EVB matrices (12.5 KB at 27 dof) are staged
+// in LDS once per workgroup, every other kind keeps its few parameters in global memory.
+template <int KIND, int MAXD>
+struct StagedPes {
+  static constexpr int kLds = KIND == PES_EVB ? evb_dev_size<MAXD>() : 1;
+  __device__ static PesDev stage(const PesDev& P, double* lds) {
+    PesDev Q = P;
+    if constexpr (KIND == PES_EVB) {
+      for (int e = threadIdx.x; e < kLds; e += blockDim.x) lds[e] = P.dp[e];
+      Q.dp = lds;
+    }
+    __syncthreads();
+    return Q;
+  }
+};
+
 // U_M and its adjoint gradient (UMforceenergy, instantonmod.f90:93-151):
 // U = sum_i V(x_i) + sum_{i<n} (m/2bn^2)|x_{i+1}-x_i|^2 [+ (m/2bn^2)(|x_1-a|^2 + |b-x_n|^2)]
+// Grid (bead tiles, chains), one bead per thread; each workgroup adds its partial sum to
+// energy[c], which the host clears before the launch.
 template <int KIND, int MAXD>
-__global__ void __launch_bounds__(kRingThreads) ring_um_kernel(RingArgs A, const double* __restrict__ x,
-                                                               double* __restrict__ grad,
-                                                               double* __restrict__ energy) {
-  __shared__ double red[kRingThreads];
-  const int c = blockIdx.x;
+__global__ void __launch_bounds__(kRingTile) ring_um_kernel(RingArgs A, const double* __restrict__ x,
+                                                            double* __restrict__ grad,
+                                                            double* __restrict__ energy) {
+  __shared__ double red[kRingTile];
+  __shared__ double pl[StagedPes<KIND, MAXD>::kLds];
+  const int c = blockIdx.y;
   const int n = A.n[c], D = A.D;
+  const int i = blockIdx.x * kRingTile + threadIdx.x;
+  if (blockIdx.x * kRingTile >= n) return;  // the same for the whole workgroup
+  const PesDev P = StagedPes<KIND, MAXD>::stage(A.pes, pl);
   const double bn2 = A.betan[c] * A.betan[c];
   const double* xc = x + A.off[c];
   const double* a = A.ab + (size_t)c * 2 * D;
   const double* b = a + D;
   double e = 0.0;
-  for (int i = threadIdx.x; i < n; i += kRingThreads) {
+  if (i < n) {
     double xs[MAXD], gs[MAXD];
 #pragma unroll
     for (int d = 0; d < MAXD; ++d) xs[d] = (d < D) ? xc[(size_t)d * n + i] : 0.0;
-    e += eval_kind<KIND, MAXD>(A.pes, xs, gs);
+    e += eval_kind<KIND, MAXD>(P, xs, gs);
     for (int d = 0; d < D; ++d) {
       const double m = A.massd[d], xi = xs[d];
       const double* xd = xc + (size_t)d * n;
@@ -67,7 +91,9 @@ __global__ void __launch_bounds__(kRingThreads) ring_um_kernel(RingArgs A, const
         e += (0.5 * m / bn2) * dx * dx;
       }
       double sg;
-      if (i == 0)
+      if (n == 1)
+        sg = A.fixedends ? m * (2.0 * xi - a[d] - b[d]) / bn2 : 0.0;
+      else if (i == 0)
         sg = A.fixedends ? m * (2.0 * xi - a[d] - xd[1]) / bn2 : m * (xi - xd[1]) / bn2;
       else if (i == n - 1)
         sg = A.fixedends ? m * (2.0 * xi - xd[n - 2] - b[d]) / bn2 : m * (xi - xd[n - 2]) / bn2;
@@ -78,37 +104,45 @@ __global__ void __launch_bounds__(kRingThreads) ring_um_kernel(RingArgs A, const
       if (grad) grad[A.off[c] + (size_t)d * n + i] = sg + gs[d];
     }
   }
-  const double tot = block_sum(e, red);
-  if (threadIdx.x == 0) energy[c] = tot;
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int st = kRingTile / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(&energy[c], red[0]);
 }
 
 // Hessian-vector product of U_M: springs exactly, potential by forward-over-reverse.
 template <int KIND, int MAXD>
-__global__ void __launch_bounds__(kRingThreads) ring_hvp_kernel(RingArgs A, const double* __restrict__ x,
-                                                                const double* __restrict__ v,
-                                                                double* __restrict__ hv) {
-  const int c = blockIdx.x;
+__global__ void __launch_bounds__(kRingTile) ring_hvp_kernel(RingArgs A, const double* __restrict__ x,
+                                                             const double* __restrict__ v,
+                                                             double* __restrict__ hv) {
+  __shared__ double pl[StagedPes<KIND, MAXD>::kLds];
+  const int c = blockIdx.y;
   const int n = A.n[c], D = A.D;
+  const int i = blockIdx.x * kRingTile + threadIdx.x;
+  if (blockIdx.x * kRingTile >= n) return;  // the same for the whole workgroup
+  const PesDev P = StagedPes<KIND, MAXD>::stage(A.pes, pl);
   const double bn2 = A.betan[c] * A.betan[c];
   const int64_t o = A.off[c];
-  for (int i = threadIdx.x; i < n; i += kRingThreads) {
-    Dual xs[MAXD], gs[MAXD];
+  if (i >= n) return;
+  Dual xs[MAXD], gs[MAXD];
 #pragma unroll
-    for (int d = 0; d < MAXD; ++d)
-      xs[d] = (d < D) ? Dual(x[o + (size_t)d * n + i], v[o + (size_t)d * n + i]) : Dual(0.0);
-    (void)eval_kind<KIND, MAXD>(A.pes, xs, gs);
-    for (int d = 0; d < D; ++d) {
-      const double m = A.massd[d];
-      const double* vd = v + o + (size_t)d * n;
-      double s;
-      if (i == 0)
-        s = A.fixedends ? 2.0 * vd[0] - (n > 1 ? vd[1] : 0.0) : vd[0] - (n > 1 ? vd[1] : 0.0);
-      else if (i == n - 1)
-        s = A.fixedends ? 2.0 * vd[i] - vd[i - 1] : vd[i] - vd[i - 1];
-      else
-        s = 2.0 * vd[i] - vd[i - 1] - vd[i + 1];
-      hv[o + (size_t)d * n + i] = m * s / bn2 + gs[d].d;
-    }
+  for (int d = 0; d < MAXD; ++d)
+    xs[d] = (d < D) ? Dual(x[o + (size_t)d * n + i], v[o + (size_t)d * n + i]) : Dual(0.0);
+  (void)eval_kind<KIND, MAXD>(P, xs, gs);
+  for (int d = 0; d < D; ++d) {
+    const double m = A.massd[d];
+    const double* vd = v + o + (size_t)d * n;
+    double s;
+    if (i == 0)
+      s = A.fixedends ? 2.0 * vd[0] - (n > 1 ? vd[1] : 0.0) : vd[0] - (n > 1 ? vd[1] : 0.0);
+    else if (i == n - 1)
+      s = A.fixedends ? 2.0 * vd[i] - vd[i - 1] : vd[i] - vd[i - 1];
+    else
+      s = 2.0 * vd[i] - vd[i - 1] - vd[i + 1];
+    hv[o + (size_t)d * n + i] = m * s / bn2 + gs[d].d;
   }
 }
 
@@ -117,8 +151,11 @@ __global__ void __launch_bounds__(kRingThreads) ring_hvp_kernel(RingArgs A, cons
 template <int KIND, int MAXD>
 __global__ void ring_hblock_kernel(RingArgs A, const double* __restrict__ x, int single_first,
                                    double* __restrict__ blocks) {
+  __shared__ double pl[StagedPes<KIND, MAXD>::kLds];
   const int c = blockIdx.y;
   const int n = A.n[c], D = A.D;
+  if ((int)(blockIdx.x * blockDim.x) >= n * D) return;  // the same for the whole workgroup
+  const PesDev P = StagedPes<KIND, MAXD>::stage(A.pes, pl);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int bead = t / D, i = t - bead * D;
   if (bead >= n) return;
@@ -128,7 +165,7 @@ __global__ void ring_hblock_kernel(RingArgs A, const double* __restrict__ x, int
 #pragma unroll
   for (int d = 0; d < MAXD; ++d)
     xs[d] = (d < D) ? Dual(x[o + (size_t)d * n + src], d == i ? 1.0 : 0.0) : Dual(0.0);
-  (void)eval_kind<KIND, MAXD>(A.pes, xs, gs);
+  (void)eval_kind<KIND, MAXD>(P, xs, gs);
   double* blk = blocks + ((size_t)(A.boff[c] + bead) * D + i) * D;
   for (int k = 0; k < D; ++k) blk[k] = gs[k].d;
 }
@@ -373,6 +410,7 @@ struct rp_ring {
   std::vector<double> betan;
   std::vector<int64_t> off, boff;
   int64_t tot = 0, nbeads = 0;
+  int nmax = 0;  // the longest chain
   hipStream_t s = nullptr;
   DevBuf<double> d_x, d_g, d_v, d_hv, d_e, d_ab, d_betan, d_massd, d_blocks, d_sinv, d_work, d_ld,
       d_eta;
@@ -400,8 +438,10 @@ static RingArgs ring_args(rp_ring* r) {
 namespace rp {
 int ring_eval(rp_ring* r, bool want_grad) {
   RingArgs a = ring_args(r);
+  RP_HIP(hipMemsetAsync(r->d_e.p, 0, sizeof(double) * r->nchain, r->s));
+  const dim3 grid((unsigned)((r->nmax + kRingTile - 1) / kRingTile), (unsigned)r->nchain);
   return dispatch_pes(r->pes, [&]<int K, int M>() -> int {
-    ring_um_kernel<K, M><<<r->nchain, kRingThreads, 0, r->s>>>(a, r->d_x.p,
+    ring_um_kernel<K, M><<<grid, kRingTile, 0, r->s>>>(a, r->d_x.p,
                                                               want_grad ? r->d_g.p : nullptr, r->d_e.p);
     RP_HIP(hipGetLastError());
     return RP_OK;
@@ -451,6 +491,7 @@ extern "C" int rp_ring_create(rp_pes* pes, const double* mass, int nchain, const
     r->boff.push_back(r->nbeads);
     r->tot += (int64_t)n[c] * D;
     r->nbeads += n[c];
+    r->nmax = std::max(r->nmax, n[c]);
     if (a) memcpy(ab.data() + (size_t)c * 2 * D, a + (size_t)c * D, D * sizeof(double));
     if (b) memcpy(ab.data() + (size_t)c * 2 * D + D, b + (size_t)c * D, D * sizeof(double));
   }
@@ -524,7 +565,8 @@ extern "C" int rp_ring_hvp(rp_ring* r, const double* v, double* hv) {
   if ((rc = r->d_v.upload(v, r->tot, r->s))) return rc;
   RingArgs a = ring_args(r);
   rc = dispatch_pes(r->pes, [&]<int K, int M>() -> int {
-    ring_hvp_kernel<K, M><<<r->nchain, kRingThreads, 0, r->s>>>(a, r->d_x.p, r->d_v.p, r->d_hv.p);
+    const dim3 grid((unsigned)((r->nmax + kRingTile - 1) / kRingTile), (unsigned)r->nchain);
+    ring_hvp_kernel<K, M><<<grid, kRingTile, 0, r->s>>>(a, r->d_x.p, r->d_v.p, r->d_hv.p);
     RP_HIP(hipGetLastError());
     return RP_OK;
   });
