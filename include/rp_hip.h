@@ -158,6 +158,13 @@ int rp_batch_run(rp_batch *b, int64_t nsteps, int64_t imin, const double *noise,
 int rp_batch_ti_readout(rp_batch *b, double *dhdr_sum, int64_t *count, int64_t *nreinit);
 /* per polymer: Philox draw cursor (the stream position of its next normal) */
 int rp_batch_get_cursor(rp_batch *b, int64_t *cursor);
+/* per polymer: Philox cursor of the Andersen Poisson intervals */
+int rp_batch_get_kick_cursor(rp_batch *b, int64_t *kick_cursor);
+/* resume from a restart file (pimd_par.f90:355-370): set the normal (cursor) and Poisson
+ * (kick_cursor) stream positions per polymer; either may be NULL to keep it.  A new
+ * kick_cursor discards the Poisson interval in progress, as propagate_pimd_nm draws a fresh
+ * one on entry (verletmodule.f90:201). */
+int rp_batch_set_cursor(rp_batch *b, const int64_t *cursor, const int64_t *kick_cursor);
 /* families of equal (n, betan), one fused-kernel launch each: n[f], count[f] */
 int rp_batch_families(rp_batch *b, int *nfam, int *n, int *count);
 /* launch shape: steps per launch (>= 1; 0 keeps the current value) and waves per

@@ -578,6 +578,43 @@ extern "C" int rp_batch_get_cursor(rp_batch* b, int64_t* cursor) {
   return RP_OK;
 }
 
+extern "C" int rp_batch_get_kick_cursor(rp_batch* b, int64_t* kick_cursor) {
+  RP_REQUIRE(b && kick_cursor, "rp_batch_get_kick_cursor: bad arguments");
+  std::vector<int64_t> pi((size_t)b->npoly * kPI64);
+  int rc;
+  if ((rc = b->d_pi64.download(pi.data(), pi.size(), b->s))) return rc;
+  RP_HIP(hipStreamSynchronize(b->s));
+  for (int p = 0; p < b->npoly; ++p)
+    kick_cursor[p] = pi[(size_t)b->pos[p] * kPI64 + PI_KCURSOR];
+  return RP_OK;
+}
+
+// Resume: place every polymer's normal and Poisson draws at saved stream positions.  A new
+// Poisson cursor also discards the interval in progress, so the next step draws a fresh one,
+// as propagate_pimd_nm does on entry.
+extern "C" int rp_batch_set_cursor(rp_batch* b, const int64_t* cursor, const int64_t* kick_cursor) {
+  RP_REQUIRE(b, "rp_batch_set_cursor: NULL batch");
+  for (int p = 0; p < b->npoly; ++p)
+    RP_REQUIRE((!cursor || cursor[p] >= 0) && (!kick_cursor || kick_cursor[p] >= 0),
+               "rp_batch_set_cursor: negative cursor");
+  std::vector<int64_t> pi((size_t)b->npoly * kPI64);
+  int rc;
+  if ((rc = b->d_pi64.download(pi.data(), pi.size(), b->s))) return rc;
+  RP_HIP(hipStreamSynchronize(b->s));
+  for (int p = 0; p < b->npoly; ++p) {
+    int64_t* q = pi.data() + (size_t)b->pos[p] * kPI64;
+    if (cursor) q[PI_CURSOR] = cursor[p];
+    if (kick_cursor) {
+      q[PI_KCURSOR] = kick_cursor[p];
+      q[PI_RKICK] = -1;
+      q[PI_ACOUNT] = 0;
+    }
+  }
+  if ((rc = b->d_pi64.upload(pi.data(), pi.size(), b->s))) return rc;
+  RP_HIP(hipStreamSynchronize(b->s));
+  return RP_OK;
+}
+
 // 8 waves per workgroup needs at most 16 MFMA units (2 per wave)
 static bool eight_waves_fit(const rp_batch* b, const Family& f) {
   return (f.NPAD >> 5) * b->DT <= 16;

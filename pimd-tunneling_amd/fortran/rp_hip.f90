@@ -259,6 +259,18 @@ module rp_hip
        type(c_ptr), value :: b
        integer(c_int64_t), intent(out) :: cursor(*)
      end function rp_batch_get_cursor
+     integer(c_int) function rp_batch_get_kick_cursor(b, kick_cursor) &
+          bind(C, name="rp_batch_get_kick_cursor")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: b
+       integer(c_int64_t), intent(out) :: kick_cursor(*)
+     end function rp_batch_get_kick_cursor
+     ! cursor, kick_cursor: c_loc of int64 arrays, or c_null_ptr to keep the current values
+     integer(c_int) function rp_batch_set_cursor(b, cursor, kick_cursor) &
+          bind(C, name="rp_batch_set_cursor")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: b, cursor, kick_cursor
+     end function rp_batch_set_cursor
      integer(c_int) function rp_batch_families(b, nfam, n, count) bind(C, name="rp_batch_families")
        import :: c_int, c_ptr
        type(c_ptr), value :: b, n, count

@@ -79,6 +79,8 @@ _SIGS = {
     "rp_batch_run": [vp, C.c_int64, C.c_int64, dp, C.c_int64],
     "rp_batch_set_kicks": [vp, i32p, C.c_int64],
     "rp_batch_get_cursor": [vp, i64p],
+    "rp_batch_get_kick_cursor": [vp, i64p],
+    "rp_batch_set_cursor": [vp, i64p, i64p],
     "rp_batch_families": [vp, ip, ip, ip],
     "rp_batch_ti_readout": [vp, dp, i64p, i64p],
     "rp_batch_set_launch": [vp, C.c_int, C.c_int],

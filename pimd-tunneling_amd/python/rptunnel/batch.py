@@ -93,6 +93,22 @@ class Batch:
         L.check(L.lib().rp_batch_get_cursor(self.h, c.ctypes.data_as(L.i64p)))
         return c
 
+    def kick_cursors(self):
+        c = np.zeros(self.npoly, dtype=np.int64)
+        L.check(L.lib().rp_batch_get_kick_cursor(self.h, c.ctypes.data_as(L.i64p)))
+        return c
+
+    def set_cursors(self, cursor=None, kick_cursor=None):
+        """Stream positions of every polymer (None keeps them), e.g. from restart files."""
+        c = None if cursor is None else np.ascontiguousarray(cursor, dtype=np.int64)
+        k = None if kick_cursor is None else np.ascontiguousarray(kick_cursor, dtype=np.int64)
+        for a in (c, k):
+            if a is not None and a.shape != (self.npoly,):
+                raise ValueError(f"set_cursors: need {self.npoly} cursors, got {a.shape}")
+        L.check(L.lib().rp_batch_set_cursor(
+            self.h, None if c is None else c.ctypes.data_as(L.i64p),
+            None if k is None else k.ctypes.data_as(L.i64p)))
+
     def families(self):
         nf = C.c_int(0)
         n = np.zeros(64, dtype=np.int32)

@@ -152,6 +152,7 @@ def linear_path(well1, well2, npath=2):
 
 
 def arc_lampath(path):
-    d = np.linalg.norm(np.diff(path, axis=1), axis=0)
-    lam = np.concatenate([[0.0], np.cumsum(d)])
-    return lam / lam[-1]
+    """Normalised arc length of a path stored (ndof, npath)."""
+    from .inputs import arc_lampath as frames_lampath
+
+    return frames_lampath(np.asarray(path, dtype=np.float64).T)

@@ -99,10 +99,7 @@ def run_ti(batch_factory, spec, n, beta, nintegral, nrep, nsteps, imin, path=Non
         pth = np.asarray(path, dtype=np.float64)
         lampath = arc_lampath(pth)
     xi, w = gauleg(0.0, 1.0, nintegral)
-    from .specs import PesSpec  # noqa: F401
-
     # endpoints b(xi) and db/dxi from the path spline (pimd_par.f90:214-221)
-    from ._lib import lib  # noqa: F401
     ends, grads = _spline_nodes(lampath, pth, xi)
     start = pth[:, 0]
     polymers, keep = [], []
@@ -129,59 +126,12 @@ def run_ti(batch_factory, spec, n, beta, nintegral, nrep, nsteps, imin, path=Non
 
 
 def _spline_nodes(lampath, path, xi):
-    """splint / splin_grad of every dof at the nodes (instantonmod.f90:499-561)."""
-    npath = len(lampath)
-    ndof = path.shape[0]
-    y2 = np.zeros_like(path)
-    for d in range(ndof):
-        y2[d] = _natural_spline(lampath, path[d])
-    ends = np.zeros((len(xi), ndof))
-    grads = np.zeros((len(xi), ndof))
-    for i, x in enumerate(xi):
-        klo = max(min(_locate(lampath, x), npath - 1), 1) - 1
-        khi = klo + 1
-        h = lampath[khi] - lampath[klo]
-        a = (lampath[khi] - x) / h
-        b = (x - lampath[klo]) / h
-        ends[i] = a * path[:, klo] + b * path[:, khi] + ((a ** 3 - a) * y2[:, klo] + (b ** 3 - b) * y2[:, khi]) * h * h / 6.0
-        grads[i] = (path[:, khi] - path[:, klo]) / h + ((1.0 - 3.0 * a * a) * y2[:, klo] + (3.0 * b * b - 1.0) * y2[:, khi]) * h / 6.0
+    """splint / splin_grad of every dof at the nodes (instantonmod.f90:499-561);
+    path is (ndof, npath)."""
+    from .inputs import spline_columns, splin_grad, splint
+
+    P = np.ascontiguousarray(np.asarray(path, dtype=np.float64).T)
+    y2 = spline_columns(lampath, P)
+    ends = np.stack([splint(lampath, P, y2, x) for x in xi])
+    grads = np.stack([splin_grad(lampath, P, y2, x) for x in xi])
     return ends, grads
-
-
-def _locate(xx, x):
-    n = len(xx)
-    ascnd = xx[-1] >= xx[0]
-    jl, ju = 0, n + 1
-    while ju - jl > 1:
-        jm = (ju + jl) // 2
-        if ascnd == (x >= xx[jm - 1]):
-            jl = jm
-        else:
-            ju = jm
-    if x == xx[0]:
-        return 1
-    if x == xx[-1]:
-        return n - 1
-    return jl
-
-
-def _natural_spline(x, y):
-    n = len(x)
-    a, b, c, r, gam, y2 = (np.zeros(n) for _ in range(6))
-    c[: n - 1] = x[1:] - x[:-1]
-    r[: n - 1] = 6.0 * ((y[1:] - y[:-1]) / c[: n - 1])
-    r[1 : n - 1] = r[1 : n - 1] - r[: n - 2].copy()
-    a[1 : n - 1] = c[: n - 2]
-    b[1 : n - 1] = 2.0 * (c[1 : n - 1] + a[1 : n - 1])
-    b[0] = b[n - 1] = 1.0
-    r[0] = c[0] = 0.0
-    r[n - 1] = a[n - 1] = 0.0
-    bet = b[0]
-    y2[0] = r[0] / bet
-    for j in range(1, n):
-        gam[j] = c[j - 1] / bet
-        bet = b[j] - a[j] * gam[j]
-        y2[j] = (r[j] - a[j] * y2[j - 1]) / bet
-    for j in range(n - 2, -1, -1):
-        y2[j] = y2[j] - gam[j + 1] * y2[j + 1]
-    return y2
