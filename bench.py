@@ -140,7 +140,9 @@ def instanton_sample(pes, sp, nchain=256, reps=10, maxiter=300, seed=4):
     log-determinant bead blocks per second (lndetJ0 at well 1), and one batched L-BFGS
     run from straight lines; each figure includes the host copies of its inputs/outputs
     (the gradient and HVP figures move n*ndof doubles each way per call), except the
-    energy-only U_M figure, which returns one double per chain."""
+    energy-only U_M figure, which returns one double per chain.  The L-BFGS search keeps
+    its vectors on the device: all chains iterate in lockstep, so its wall time follows
+    the largest iteration count."""
     from rptunnel import Ring
 
     rng = np.random.default_rng(seed)
@@ -187,6 +189,7 @@ def instanton_sample(pes, sp, nchain=256, reps=10, maxiter=300, seed=4):
             "hvp_bead_evals_per_s": beads / t_hvp,
             "logdet_bead_blocks_per_s": beads / t_ld,
             "lbfgs": {"seconds": t_opt, "max_iterations": int(np.max(it)),
+                      "ms_per_iteration": 1e3 * t_opt / max(int(np.max(it)), 1),
                       "mean_iterations": float(np.mean(it)),
                       "converged": int(np.sum(st == 0)), "chains": nchain,
                       "pgtol": 1e-5, "factr": 1e6, "iteration_cap": maxiter}}

@@ -207,9 +207,11 @@ int rp_ring_hessian_band(rp_ring *r, int chain, int singlewell, double *band);
  * the inertia (two or more negative eigenvalues). */
 int rp_ring_logdet(rp_ring *r, int singlewell, int skip_lowest, double *logdet_pos,
                    int *nneg, double *eta1);
-/* instanton (:679-777): L-BFGS-B (m corrections, unbounded as nbd=0) on every chain at
- * once, gradients from the device adjoint.  x_inout as rp_ring_set_x; action[c] = U_M at
- * the optimum; iters[c]; status[c] = 0 converged, 1 abnormal line search, 2 iteration cap. */
+/* instanton (:679-777): L-BFGS-B (m <= 32 corrections, unbounded as nbd=0) on every chain
+ * at once, resident on the device (iterates, gradients and correction pairs stay in HBM;
+ * gradients from the adjoint).  x_inout as rp_ring_set_x; action[c] = U_M at the optimum;
+ * iters[c]; status[c] = 0 converged, 1 abnormal line search, 2 iteration cap (maxiter, or
+ * 200000 evaluations when maxiter = 0).  The optimum is left as the ring's x. */
 int rp_instanton(rp_ring *r, double pgtol, double factr, int m, int maxiter, double *x_inout,
                  double *action, int *iters, int *status);
 /* adjoint sensitivity dU_M/dtheta of each chain to the potential parameters

@@ -1,7 +1,5 @@
 // rp_instanton.hip -- the instanton search (instantonmod.f90:679-777) for a batch of
-// chains: one quasi-Newton state machine per chain on the host, every function and
-// gradient request of all chains served by one device evaluation of U_M and its adjoint
-// gradient (ring_um_kernel).
+// chains, resident on the device.
 //
 // The reference drives L-BFGS-B (Byrd, Lu, Nocedal, Zhu 1995; v3.0 with Morales-Nocedal
 // 2011) with nbd = 0, i.e. no bounds, m = 8 corrections, factr = 1e6 and pgtol = eps2.
@@ -11,12 +9,19 @@
 // Alg. 7.4; strong-Wolfe line search with zoom, Alg. 3.5/3.6) with the reference's two
 // stopping tests:
 //   (f_k - f_{k+1}) / max(|f_k|, |f_{k+1}|, 1) <= factr * epsmch,   |g|_inf <= pgtol.
+//
+// Every vector of every chain (iterate, gradient, direction, the gradients kept by the line
+// search, the m correction pairs) lives in HBM; one workgroup per chain runs the chain's
+// line-search state machine on scalars that every lane holds identically, and the vector
+// work (dot products, the two-loop recursion, trial points) over the chain's coordinates.
+// One search round is three launches for all chains at once: the ring's U_M + adjoint
+// gradient at every trial point, lb_feed_kernel (line-search step or acceptance and the
+// stopping tests), lb_direction_kernel (two-loop recursion and the first trial point of the
+// next iteration).  The host only launches rounds and reads a status word per chain every
+// few rounds; no coordinates cross PCIe during the search.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
-#include <cstring>
-#include <functional>
-#include <thread>
 #include <vector>
 
 #include "rp_ring.h"
@@ -25,279 +30,446 @@ using namespace rp;
 
 namespace {
 
-enum Phase { LS_BRACKET, LS_ZOOM };
-enum Status { ST_RUN = -1, ST_CONV = 0, ST_ABNORMAL = 1, ST_MAXITER = 2 };
+enum : int { LS_BRACKET = 0, LS_ZOOM = 1 };
+enum : int { ST_RUN = -1, ST_CONV = 0, ST_ABNORMAL = 1, ST_MAXITER = 2 };
+enum : int { OP_NONE = 0, OP_DIRECTION = 1 };
+enum : int { ACT_NONE = 0, ACT_TRIAL = 1, ACT_ACCEPT_CUR = 2, ACT_ACCEPT_LO = 3 };
+// gradient copies that go with a line-search step
+enum : int { CP_GT_TO_PREV = 1, CP_GT_TO_LO = 2, CP_PREV_TO_LO = 4, CP_G_TO_LO = 8 };
 
-struct Chain {
-  int n = 0;                       // number of coordinates of this chain
-  std::vector<double> x, g, d, xt;  // iterate, gradient, direction, trial point
-  std::vector<std::vector<double>> S, Y;
-  std::vector<double> rho;
-  double f = 0.0;
-  int iter = 0, status = ST_RUN, nls = 0;
-  // line search
-  Phase ph = LS_BRACKET;
-  double a_cur = 0, a_prev = 0, f_prev = 0, dg_prev = 0;
-  double a_lo = 0, f_lo = 0, dg_lo = 0, a_hi = 0, f_hi = 0, dg_hi = 0;
-  double f0 = 0, dg0 = 0;
-  std::vector<double> g_lo, g_prev;  // gradients at a_lo and a_prev
+constexpr int kMaxM = 32;         // largest number of corrections
+constexpr int kLbThreads = 1024;  // one workgroup of 16 waves per chain
+constexpr int kLbWaves = kLbThreads / 64;
+constexpr double kC1 = 1e-3;      // sufficient decrease (L-BFGS-B's ftol)
+constexpr double kC2 = 0.9;       // curvature (L-BFGS-B's gtol)
+constexpr int kMaxLS = 40;        // line-search evaluations (the reference's maxiter argument)
+constexpr int kCheckEvery = 4;    // rounds between status reads
+constexpr long kMaxRounds = 200000;  // evaluations when the caller sets no iteration cap
+
+// Per chain scalar state.  The correction pairs sit in a ring of m + 1 slots: the history is
+// `count` pairs from slot `head`, and slot (head + count) % (m + 1) is always free for the
+// candidate pair of an acceptance, so a rejected pair never overwrites a kept one.  The
+// gradient at a_prev is the iterate's own gradient G while prev_is_g is set (a_prev = 0),
+// else the saved copy Gprev.
+struct LbChain {
+  double f, f0, dg0, a_cur, a_prev, f_prev, dg_prev, a_lo, f_lo, dg_lo, a_hi, f_hi, dg_hi;
+  double ginf, gam;
+  int iter, status, nls, phase, count, head, op, prev_is_g;
 };
 
-constexpr double kC1 = 1e-3;   // sufficient decrease (L-BFGS-B's ftol)
-constexpr double kC2 = 0.9;    // curvature (L-BFGS-B's gtol)
-constexpr int kMaxLS = 40;     // line-search evaluations (the reference's maxiter argument)
+struct LbArgs {
+  int nchain, m, maxiter, D;
+  double factr, pgtol;
+  int64_t tot;
+  const int64_t* off;
+  const int* n;
+  double *X, *G, *Dv, *Glo, *Gprev, *S, *Y;  // iterate, gradient, direction, saved gradients, pairs
+  double *XT, *GT;                            // the ring's evaluation point and gradient
+  const double* E;                            // the ring's energies
+  LbChain* st;
+  double* rho;                                // 1/(s.y) per chain and slot, (kMaxM + 1) each
+  int* status;                                // status word per chain for the host
+};
 
-double dot(const std::vector<double>& a, const std::vector<double>& b) {
-  double s = 0.0;
-  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
-  return s;
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+// workgroup reductions in a fixed order (deterministic); red holds kLbWaves doubles
+__device__ double lb_sum(double v, double* red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < kLbWaves; ++w) t += red[w];
+  return t;
+}
+__device__ double lb_max(double v, double* red) {
+  v = wave_max(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < kLbWaves; ++w) t = fmax(t, red[w]);
+  return t;
 }
 
-double infnorm(const std::vector<double>& a) {
-  double s = 0.0;
-  for (double v : a) s = std::max(s, std::fabs(v));
-  return s;
-}
-
-// d = -H g by the two-loop recursion
-void direction(Chain& c) {
-  const int k = (int)c.S.size();
-  c.d = c.g;
-  std::vector<double> alpha(k);
-  for (int i = k - 1; i >= 0; --i) {
-    alpha[i] = c.rho[i] * dot(c.S[i], c.d);
-    for (int j = 0; j < c.n; ++j) c.d[j] -= alpha[i] * c.Y[i][j];
-  }
-  double gamma = 1.0;
-  if (k > 0) gamma = dot(c.S[k - 1], c.Y[k - 1]) / dot(c.Y[k - 1], c.Y[k - 1]);
-  for (double& v : c.d) v *= gamma;
-  for (int i = 0; i < k; ++i) {
-    const double beta = c.rho[i] * dot(c.Y[i], c.d);
-    for (int j = 0; j < c.n; ++j) c.d[j] += c.S[i][j] * (alpha[i] - beta);
-  }
-  for (double& v : c.d) v = -v;
-}
-
-double cubic_min(double a, double fa, double da, double b, double fb, double db) {
+__device__ double cubic_min(double a, double fa, double da, double b, double fb, double db) {
   // minimiser of the cubic interpolating (a, fa, da), (b, fb, db), safeguarded
   const double d1 = da + db - 3.0 * (fa - fb) / (a - b);
   const double disc = d1 * d1 - da * db;
   double t;
   if (disc >= 0.0) {
-    const double d2 = std::copysign(std::sqrt(disc), b - a);
+    const double d2 = copysign(sqrt(disc), b - a);
     t = b - (b - a) * (db + d2 - d1) / (db - da + 2.0 * d2);
   } else {
     t = 0.5 * (a + b);
   }
-  const double lo = std::min(a, b), hi = std::max(a, b), w = hi - lo;
+  const double lo = fmin(a, b), hi = fmax(a, b), w = hi - lo;
   if (!(t > lo + 0.1 * w && t < hi - 0.1 * w)) t = 0.5 * (a + b);
   return t;
 }
 
-void set_trial(Chain& c, double a) {
-  c.a_cur = a;
-  for (int j = 0; j < c.n; ++j) c.xt[j] = c.x[j] + a * c.d[j];
+// first function and gradient: X = x0, G = g(x0), convergence of the start point
+__global__ void __launch_bounds__(kLbThreads) lb_init_kernel(LbArgs A) {
+  __shared__ double red[kLbWaves];
+  const int c = blockIdx.x;
+  const int64_t base = A.off[c];
+  const int L = A.n[c] * A.D;
+  double gm = 0.0;
+  for (int j = threadIdx.x; j < L; j += kLbThreads) {
+    const double g = A.GT[base + j];
+    A.X[base + j] = A.XT[base + j];
+    A.G[base + j] = g;
+    gm = fmax(gm, fabs(g));
+  }
+  const double ginf = lb_max(gm, red);
+  if (threadIdx.x == 0) {
+    LbChain s = {};
+    s.f = A.E[c];
+    s.ginf = ginf;
+    s.gam = 1.0;
+    s.status = ginf <= A.pgtol ? ST_CONV : ST_RUN;
+    s.op = s.status == ST_RUN ? OP_DIRECTION : OP_NONE;
+    A.st[c] = s;
+    A.status[c] = s.status;
+  }
 }
 
-// begin an iteration: direction + first trial step
-void start_iteration(Chain& c) {
-  direction(c);
-  c.dg0 = dot(c.g, c.d);
-  if (!(c.dg0 < 0.0)) {  // not a descent direction: restart from steepest descent
-    c.S.clear();
-    c.Y.clear();
-    c.rho.clear();
-    c.d = c.g;
-    for (double& v : c.d) v = -v;
-    c.dg0 = dot(c.g, c.d);
-  }
-  c.f0 = c.f;
-  c.ph = LS_BRACKET;
-  c.a_prev = 0.0;
-  c.f_prev = c.f;
-  c.dg_prev = c.dg0;
-  c.g_prev = c.g;
-  c.nls = 0;
-  const double a0 = c.S.empty() ? std::min(1.0, 1.0 / std::max(infnorm(c.g), 1e-300)) : 1.0;
-  set_trial(c, a0);
-}
-
-// accept the trial point (f, g) and test convergence; returns false when finished
-bool accept(Chain& c, double ft, const std::vector<double>& gt, double factr, double pgtol, int m,
-            int maxiter) {
-  std::vector<double> s(c.n), y(c.n);
-  for (int j = 0; j < c.n; ++j) {
-    s[j] = c.xt[j] - c.x[j];
-    y[j] = gt[j] - c.g[j];
-  }
-  const double sy = dot(s, y);
-  if (sy > DBL_EPSILON * dot(y, y)) {
-    if ((int)c.S.size() == m) {
-      c.S.erase(c.S.begin());
-      c.Y.erase(c.Y.begin());
-      c.rho.erase(c.rho.begin());
+// d = -H g by the two-loop recursion, the descent check, and the first trial point.  Each
+// pass over the chain fuses one correction's update with the next correction's dot product,
+// so the recursion reads the direction 2k + 1 times for k pairs.
+__global__ void __launch_bounds__(kLbThreads) lb_direction_kernel(LbArgs A) {
+  __shared__ double red[kLbWaves];
+  __shared__ double alpha[kMaxM + 1];
+  const int c = blockIdx.x;
+  if (A.st[c].op != OP_DIRECTION) return;  // uniform over the workgroup
+  const int k = A.st[c].count, head = A.st[c].head;
+  const double gam = A.st[c].gam, ginf = A.st[c].ginf;
+  const double* rho = A.rho + (int64_t)c * (kMaxM + 1);
+  const int64_t base = A.off[c], tot = A.tot;
+  const int L = A.n[c] * A.D;
+  const int m1 = A.m + 1;
+  double* D = A.Dv + base;
+  const double* G = A.G + base;
+  const double* X = A.X + base;
+  double* XT = A.XT + base;
+  const double* S0 = A.S + base;
+  const double* Y0 = A.Y + base;
+  auto slot = [=](int i) { return (head + i) % m1; };
+  auto Sv = [=](int i) { return S0 + (int64_t)slot(i) * tot; };
+  auto Yv = [=](int i) { return Y0 + (int64_t)slot(i) * tot; };
+  double part = 0.0, a0 = 1.0;
+  if (k == 0) {
+    a0 = fmin(1.0, 1.0 / fmax(ginf, 1e-300));
+    for (int j = threadIdx.x; j < L; j += kLbThreads) {
+      const double v = -G[j];
+      D[j] = v;
+      part += G[j] * v;
+      XT[j] = X[j] + a0 * v;
     }
-    c.S.push_back(s);
-    c.Y.push_back(y);
-    c.rho.push_back(1.0 / sy);
-  }
-  const double fold = c.f;
-  c.x = c.xt;
-  c.g = gt;
-  c.f = ft;
-  ++c.iter;
-  if ((fold - ft) / std::max(std::max(std::fabs(fold), std::fabs(ft)), 1.0) <= factr * DBL_EPSILON ||
-      infnorm(c.g) <= pgtol) {
-    c.status = ST_CONV;
-    return false;
-  }
-  if (maxiter > 0 && c.iter >= maxiter) {
-    c.status = ST_MAXITER;
-    return false;
-  }
-  start_iteration(c);
-  return true;
-}
-
-// feed the value at the current trial point; either request another point or accept
-void feed(Chain& c, double ft, const std::vector<double>& gt, double factr, double pgtol, int m,
-          int maxiter) {
-  const double dgt = dot(gt, c.d);
-  ++c.nls;
-  const double a = c.a_cur;
-  auto zoom_step = [&]() {
-    set_trial(c, cubic_min(c.a_lo, c.f_lo, c.dg_lo, c.a_hi, c.f_hi, c.dg_hi));
-  };
-  if (c.nls >= kMaxLS) {
-    // the line search is out of evaluations: keep the best point if it decreased f
-    if (ft < c.f0 - 1e-300) {
-      accept(c, ft, gt, factr, pgtol, m, maxiter);
-    } else if (c.ph == LS_ZOOM && c.f_lo < c.f0) {
-      set_trial(c, c.a_lo);
-      accept(c, c.f_lo, c.g_lo, factr, pgtol, m, maxiter);
-    } else {
-      c.status = ST_ABNORMAL;
-    }
-    return;
-  }
-  if (c.ph == LS_BRACKET) {
-    if (ft > c.f0 + kC1 * a * c.dg0 || (c.nls > 1 && ft >= c.f_prev)) {
-      c.ph = LS_ZOOM;
-      c.a_lo = c.a_prev; c.f_lo = c.f_prev; c.dg_lo = c.dg_prev; c.g_lo = c.g_prev;
-      c.a_hi = a; c.f_hi = ft; c.dg_hi = dgt;
-      zoom_step();
-      return;
-    }
-    if (std::fabs(dgt) <= -kC2 * c.dg0) {
-      accept(c, ft, gt, factr, pgtol, m, maxiter);
-      return;
-    }
-    if (dgt >= 0.0) {
-      c.ph = LS_ZOOM;
-      c.a_lo = a; c.f_lo = ft; c.dg_lo = dgt; c.g_lo = gt;
-      c.a_hi = c.a_prev; c.f_hi = c.f_prev; c.dg_hi = c.dg_prev;
-      zoom_step();
-      return;
-    }
-    c.a_prev = a; c.f_prev = ft; c.dg_prev = dgt; c.g_prev = gt;
-    set_trial(c, 2.0 * a);  // extrapolate
-    return;
-  }
-  // zoom
-  if (ft > c.f0 + kC1 * a * c.dg0 || ft >= c.f_lo) {
-    c.a_hi = a; c.f_hi = ft; c.dg_hi = dgt;
   } else {
-    if (std::fabs(dgt) <= -kC2 * c.dg0) {
-      accept(c, ft, gt, factr, pgtol, m, maxiter);
-      return;
+    {
+      const double* Sk = Sv(k - 1);
+      for (int j = threadIdx.x; j < L; j += kLbThreads) part += Sk[j] * G[j];
     }
-    if (dgt * (c.a_hi - c.a_lo) >= 0.0) {
-      c.a_hi = c.a_lo; c.f_hi = c.f_lo; c.dg_hi = c.dg_lo;
+    for (int i = k - 1; i >= 0; --i) {
+      const double al = rho[slot(i)] * lb_sum(part, red);
+      if (threadIdx.x == 0) alpha[i] = al;
+      const double* Yp = Yv(i);
+      const double* src = i == k - 1 ? G : D;
+      part = 0.0;
+      if (i > 0) {
+        const double* Sn = Sv(i - 1);
+        for (int j = threadIdx.x; j < L; j += kLbThreads) {
+          const double d = src[j] - al * Yp[j];
+          D[j] = d;
+          part += Sn[j] * d;
+        }
+      } else {
+        // last backward update, the initial scaling, and the first forward dot product
+        for (int j = threadIdx.x; j < L; j += kLbThreads) {
+          const double d = (src[j] - al * Yp[j]) * gam;
+          D[j] = d;
+          part += Yp[j] * d;
+        }
+      }
     }
-    c.a_lo = a; c.f_lo = ft; c.dg_lo = dgt; c.g_lo = gt;
+    __syncthreads();  // alpha[] complete before the forward loop reads it
+    for (int i = 0; i < k; ++i) {
+      const double coef = alpha[i] - rho[slot(i)] * lb_sum(part, red);
+      const double* Sp = Sv(i);
+      part = 0.0;
+      if (i < k - 1) {
+        const double* Yn = Yv(i + 1);
+        for (int j = threadIdx.x; j < L; j += kLbThreads) {
+          const double d = D[j] + coef * Sp[j];
+          D[j] = d;
+          part += Yn[j] * d;
+        }
+      } else {
+        // last forward update, the sign, the slope along d and the unit trial step
+        for (int j = threadIdx.x; j < L; j += kLbThreads) {
+          const double d = -(D[j] + coef * Sp[j]);
+          D[j] = d;
+          part += G[j] * d;
+          XT[j] = X[j] + d;
+        }
+      }
+    }
   }
-  if (std::fabs(c.a_hi - c.a_lo) <= 1e-14 * std::max(1.0, c.a_lo)) {
-    // interval collapsed: take the best point found
-    if (c.f_lo < c.f0) {
-      set_trial(c, c.a_lo);
-      accept(c, c.f_lo, c.g_lo, factr, pgtol, m, maxiter);
-    } else {
-      c.status = ST_ABNORMAL;
+  double dg0 = lb_sum(part, red);
+  const bool reset = !(dg0 < 0.0);  // not a descent direction: steepest descent, no history
+  if (reset && k > 0) {
+    a0 = fmin(1.0, 1.0 / fmax(ginf, 1e-300));
+    part = 0.0;
+    for (int j = threadIdx.x; j < L; j += kLbThreads) {
+      const double v = -G[j];
+      D[j] = v;
+      part += G[j] * v;
+      XT[j] = X[j] + a0 * v;
     }
-    return;
+    dg0 = lb_sum(part, red);
   }
-  zoom_step();
+  if (threadIdx.x == 0) {
+    LbChain t = A.st[c];
+    if (reset) {
+      t.count = 0;
+      t.head = 0;
+    }
+    t.dg0 = dg0;
+    t.f0 = t.f;
+    t.phase = LS_BRACKET;
+    t.a_prev = 0.0;
+    t.f_prev = t.f;
+    t.dg_prev = dg0;
+    t.prev_is_g = 1;
+    t.nls = 0;
+    t.a_cur = a0;
+    t.op = OP_NONE;
+    A.st[c] = t;
+  }
 }
 
-void parallel_for(int n, const std::function<void(int)>& fn) {
-  const int nt = std::max(1, std::min(n, (int)std::min(16u, std::thread::hardware_concurrency())));
-  if (nt <= 1) {
-    for (int i = 0; i < n; ++i) fn(i);
-    return;
+// one line-search evaluation: the value and slope at the trial point decide the next trial
+// point or the acceptance, which stores the correction pair and tests convergence
+__global__ void __launch_bounds__(kLbThreads) lb_feed_kernel(LbArgs A) {
+  __shared__ double red[kLbWaves];
+  const int c = blockIdx.x;
+  LbChain s = A.st[c];
+  if (s.status != ST_RUN) return;  // uniform over the workgroup
+  const int64_t base = A.off[c];
+  const int L = A.n[c] * A.D;
+  double part = 0.0;
+  for (int j = threadIdx.x; j < L; j += kLbThreads) part += A.GT[base + j] * A.Dv[base + j];
+  const double dgt = lb_sum(part, red);
+  const double ft = A.E[c];
+  // every lane runs the same scalar decision
+  int act = ACT_NONE, cp = 0;
+  double a_new = 0.0;
+  const double a = s.a_cur;
+  ++s.nls;
+  if (s.nls >= kMaxLS) {
+    // out of evaluations: keep the best point if it decreased f
+    if (ft < s.f0 - 1e-300) act = ACT_ACCEPT_CUR;
+    else if (s.phase == LS_ZOOM && s.f_lo < s.f0) act = ACT_ACCEPT_LO;
+    else s.status = ST_ABNORMAL;
+  } else if (s.phase == LS_BRACKET) {
+    if (ft > s.f0 + kC1 * a * s.dg0 || (s.nls > 1 && ft >= s.f_prev)) {
+      s.phase = LS_ZOOM;
+      s.a_lo = s.a_prev; s.f_lo = s.f_prev; s.dg_lo = s.dg_prev;
+      cp |= s.prev_is_g ? CP_G_TO_LO : CP_PREV_TO_LO;
+      s.a_hi = a; s.f_hi = ft; s.dg_hi = dgt;
+      act = ACT_TRIAL;
+      a_new = cubic_min(s.a_lo, s.f_lo, s.dg_lo, s.a_hi, s.f_hi, s.dg_hi);
+    } else if (fabs(dgt) <= -kC2 * s.dg0) {
+      act = ACT_ACCEPT_CUR;
+    } else if (dgt >= 0.0) {
+      s.phase = LS_ZOOM;
+      s.a_lo = a; s.f_lo = ft; s.dg_lo = dgt; cp |= CP_GT_TO_LO;
+      s.a_hi = s.a_prev; s.f_hi = s.f_prev; s.dg_hi = s.dg_prev;
+      act = ACT_TRIAL;
+      a_new = cubic_min(s.a_lo, s.f_lo, s.dg_lo, s.a_hi, s.f_hi, s.dg_hi);
+    } else {
+      s.a_prev = a; s.f_prev = ft; s.dg_prev = dgt; s.prev_is_g = 0; cp |= CP_GT_TO_PREV;
+      act = ACT_TRIAL;
+      a_new = 2.0 * a;  // extrapolate
+    }
+  } else {
+    bool done = false;
+    if (ft > s.f0 + kC1 * a * s.dg0 || ft >= s.f_lo) {
+      s.a_hi = a; s.f_hi = ft; s.dg_hi = dgt;
+    } else if (fabs(dgt) <= -kC2 * s.dg0) {
+      act = ACT_ACCEPT_CUR;
+      done = true;
+    } else {
+      if (dgt * (s.a_hi - s.a_lo) >= 0.0) {
+        s.a_hi = s.a_lo; s.f_hi = s.f_lo; s.dg_hi = s.dg_lo;
+      }
+      s.a_lo = a; s.f_lo = ft; s.dg_lo = dgt; cp |= CP_GT_TO_LO;
+    }
+    if (!done) {
+      if (fabs(s.a_hi - s.a_lo) <= 1e-14 * fmax(1.0, s.a_lo)) {
+        // interval collapsed: take the best point found
+        if (s.f_lo < s.f0) act = ACT_ACCEPT_LO;
+        else s.status = ST_ABNORMAL;
+      } else {
+        act = ACT_TRIAL;
+        a_new = cubic_min(s.a_lo, s.f_lo, s.dg_lo, s.a_hi, s.f_hi, s.dg_hi);
+      }
+    }
   }
-  std::vector<std::thread> th;
-  for (int t = 0; t < nt; ++t)
-    th.emplace_back([&, t]() {
-      for (int i = t; i < n; i += nt) fn(i);
-    });
-  for (auto& t : th) t.join();
+  double* X = A.X + base;
+  double* G = A.G + base;
+  double* Glo = A.Glo + base;
+  double* Gp = A.Gprev + base;
+  double* XT = A.XT + base;
+  const double* GT = A.GT + base;
+  const double* Dv = A.Dv + base;
+  for (int j = threadIdx.x; j < L; j += kLbThreads) {
+    if (cp & CP_PREV_TO_LO) Glo[j] = Gp[j];
+    if (cp & CP_G_TO_LO) Glo[j] = G[j];
+    if (cp & CP_GT_TO_LO) Glo[j] = GT[j];
+    if (cp & CP_GT_TO_PREV) Gp[j] = GT[j];
+  }
+  if (act == ACT_TRIAL) {
+    for (int j = threadIdx.x; j < L; j += kLbThreads) XT[j] = X[j] + a_new * Dv[j];
+    s.a_cur = a_new;
+  } else if (act == ACT_ACCEPT_CUR || act == ACT_ACCEPT_LO) {
+    const int m1 = A.m + 1;
+    const int slot = (s.head + s.count) % m1;
+    double* Sp = A.S + (int64_t)slot * A.tot + base;
+    double* Yp = A.Y + (int64_t)slot * A.tot + base;
+    const bool cur = act == ACT_ACCEPT_CUR;
+    const double alo = s.a_lo;
+    double psy = 0.0, pyy = 0.0, pg = 0.0;
+    for (int j = threadIdx.x; j < L; j += kLbThreads) {
+      const double xa = cur ? XT[j] : X[j] + alo * Dv[j];
+      const double ga = cur ? GT[j] : Glo[j];
+      const double sj = xa - X[j], yj = ga - G[j];
+      Sp[j] = sj;
+      Yp[j] = yj;
+      psy += sj * yj;
+      pyy += yj * yj;
+      pg = fmax(pg, fabs(ga));
+      X[j] = xa;
+      G[j] = ga;
+      XT[j] = xa;
+    }
+    const double sy = lb_sum(psy, red);
+    const double yy = lb_sum(pyy, red);
+    const double ginf = lb_max(pg, red);
+    const double fa = cur ? ft : s.f_lo;
+    if (sy > DBL_EPSILON * yy) {
+      if (threadIdx.x == 0) A.rho[(int64_t)c * (kMaxM + 1) + slot] = 1.0 / sy;
+      s.gam = sy / yy;
+      if (s.count < A.m) ++s.count;
+      else s.head = (s.head + 1) % m1;
+    }
+    const double fold = s.f;
+    s.f = fa;
+    s.ginf = ginf;
+    ++s.iter;
+    if ((fold - fa) / fmax(fmax(fabs(fold), fabs(fa)), 1.0) <= A.factr * DBL_EPSILON ||
+        ginf <= A.pgtol)
+      s.status = ST_CONV;
+    else if (A.maxiter > 0 && s.iter >= A.maxiter)
+      s.status = ST_MAXITER;
+    else
+      s.op = OP_DIRECTION;
+  }
+  if (threadIdx.x == 0) {
+    A.st[c] = s;
+    A.status[c] = s.status;
+  }
+}
+
+__global__ void lb_cap_kernel(LbArgs A) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < A.nchain && A.st[c].status == ST_RUN) {
+    A.st[c].status = ST_MAXITER;
+    A.status[c] = ST_MAXITER;
+  }
 }
 
 }  // namespace
 
 extern "C" int rp_instanton(rp_ring* r, double pgtol, double factr, int m, int maxiter,
                             double* x_inout, double* action, int* iters, int* status) {
-  RP_REQUIRE(r && x_inout && action && m > 0, "rp_instanton: bad arguments");
-  const int nc = ring_nchain(r), tot = ring_tot(r);
-  const auto& off = ring_off(r);
-  const auto& nn = ring_n(r);
-  const int D = ring_D(r);
-  std::vector<Chain> ch(nc);
-  for (int c = 0; c < nc; ++c) {
-    ch[c].n = nn[c] * D;
-    ch[c].x.assign(x_inout + off[c], x_inout + off[c] + ch[c].n);
-    ch[c].xt = ch[c].x;
-  }
-  std::vector<double> xbuf(x_inout, x_inout + tot), gbuf(tot), ebuf(nc);
+  RP_REQUIRE(r && x_inout && action && m > 0 && m <= kMaxM,
+             "rp_instanton: bad arguments (m must be 1..%d)", kMaxM);
+  const int nc = ring_nchain(r);
+  const int64_t tot = ring_tot(r);
+  const RingBufs rb = ring_bufs(r);
+  DevBuf<double> X, G, Dv, Glo, Gp, S, Y;
+  DevBuf<LbChain> st;
+  DevBuf<double> rho;
+  DevBuf<int> stat;
   int rc;
-  // initial function and gradient (instantonmod.f90:725-739)
-  if ((rc = ring_upload_x(r, xbuf.data())) || (rc = ring_eval(r, true)) ||
-      (rc = ring_download(r, ebuf.data(), gbuf.data())))
+  if ((rc = X.alloc(tot)) || (rc = G.alloc(tot)) || (rc = Dv.alloc(tot)) || (rc = Glo.alloc(tot)) ||
+      (rc = Gp.alloc(tot)) || (rc = S.alloc((size_t)(m + 1) * tot)) ||
+      (rc = Y.alloc((size_t)(m + 1) * tot)) || (rc = st.alloc(nc)) || (rc = stat.alloc(nc)) ||
+      (rc = rho.alloc((size_t)nc * (kMaxM + 1))))
     return rc;
-  int active = 0;
-  for (int c = 0; c < nc; ++c) {
-    ch[c].f = ebuf[c];
-    ch[c].g.assign(gbuf.begin() + off[c], gbuf.begin() + off[c] + ch[c].n);
-    if (infnorm(ch[c].g) <= pgtol) {
-      ch[c].status = ST_CONV;
-    } else {
-      start_iteration(ch[c]);
-      ++active;
+  LbArgs A;
+  A.nchain = nc;
+  A.m = m;
+  A.maxiter = maxiter;
+  A.D = rb.D;
+  A.factr = factr;
+  A.pgtol = pgtol;
+  A.tot = tot;
+  A.off = rb.off;
+  A.n = rb.n;
+  A.X = X.p; A.G = G.p; A.Dv = Dv.p; A.Glo = Glo.p; A.Gprev = Gp.p; A.S = S.p; A.Y = Y.p;
+  A.XT = rb.x;
+  A.GT = rb.g;
+  A.E = rb.e;
+  A.st = st.p;
+  A.rho = rho.p;
+  A.status = stat.p;
+  // initial function and gradient (instantonmod.f90:725-739)
+  if ((rc = ring_upload_x(r, x_inout)) || (rc = ring_eval(r, true))) return rc;
+  lb_init_kernel<<<nc, kLbThreads, 0, rb.s>>>(A);
+  RP_HIP(hipGetLastError());
+  lb_direction_kernel<<<nc, kLbThreads, 0, rb.s>>>(A);
+  RP_HIP(hipGetLastError());
+  std::vector<int> hs(nc);
+  const long cap = maxiter > 0 ? (long)maxiter * kMaxLS + 1 : kMaxRounds;
+  bool capped = true;
+  for (long round = 0; round < cap; ++round) {
+    if (round % kCheckEvery == 0) {
+      if ((rc = stat.download(hs.data(), nc, rb.s))) return rc;
+      RP_HIP(hipStreamSynchronize(rb.s));
+      if (std::none_of(hs.begin(), hs.end(), [](int v) { return v == ST_RUN; })) {
+        capped = false;
+        break;
+      }
     }
+    if ((rc = ring_eval(r, true))) return rc;
+    lb_feed_kernel<<<nc, kLbThreads, 0, rb.s>>>(A);
+    RP_HIP(hipGetLastError());
+    lb_direction_kernel<<<nc, kLbThreads, 0, rb.s>>>(A);
+    RP_HIP(hipGetLastError());
   }
-  while (active > 0) {
-    for (int c = 0; c < nc; ++c)
-      std::copy(ch[c].status == ST_RUN ? ch[c].xt.begin() : ch[c].x.begin(),
-                ch[c].status == ST_RUN ? ch[c].xt.end() : ch[c].x.end(), xbuf.begin() + off[c]);
-    if ((rc = ring_upload_x(r, xbuf.data())) || (rc = ring_eval(r, true)) ||
-        (rc = ring_download(r, ebuf.data(), gbuf.data())))
-      return rc;
-    parallel_for(nc, [&](int c) {
-      if (ch[c].status != ST_RUN) return;
-      std::vector<double> gt(gbuf.begin() + off[c], gbuf.begin() + off[c] + ch[c].n);
-      feed(ch[c], ebuf[c], gt, factr, pgtol, m, maxiter);
-    });
-    active = 0;
-    for (int c = 0; c < nc; ++c) active += ch[c].status == ST_RUN;
+  if (capped) {
+    lb_cap_kernel<<<(nc + 255) / 256, 256, 0, rb.s>>>(A);
+    RP_HIP(hipGetLastError());
   }
+  // results; the optimum stays on the device for the Hessian / log-determinant calls
+  std::vector<LbChain> hst(nc);
+  RP_HIP(hipMemcpyAsync(rb.x, X.p, tot * sizeof(double), hipMemcpyDeviceToDevice, rb.s));
+  if ((rc = X.download(x_inout, tot, rb.s)) || (rc = st.download(hst.data(), nc, rb.s))) return rc;
+  RP_HIP(hipStreamSynchronize(rb.s));
   for (int c = 0; c < nc; ++c) {
-    std::copy(ch[c].x.begin(), ch[c].x.end(), x_inout + off[c]);
-    std::copy(ch[c].x.begin(), ch[c].x.end(), xbuf.begin() + off[c]);
-    action[c] = ch[c].f;
-    if (iters) iters[c] = ch[c].iter;
-    if (status) status[c] = ch[c].status;
+    action[c] = hst[c].f;
+    if (iters) iters[c] = hst[c].iter;
+    if (status) status[c] = hst[c].status;
   }
-  // leave the optimum on the device for the Hessian / log-determinant calls
-  if ((rc = ring_upload_x(r, xbuf.data()))) return rc;
-  return ring_download(r, nullptr, nullptr);
+  return RP_OK;
 }

@@ -14,4 +14,16 @@ int ring_D(const rp_ring* r);
 const std::vector<int>& ring_n(const rp_ring* r);
 const std::vector<int64_t>& ring_off(const rp_ring* r);
 int ring_lowest_eigen(rp_ring* r, double* eta);
+// device views used by the instanton search: the evaluation point x, its adjoint gradient
+// g and energies e (written by ring_eval), chain offsets and bead counts, the stream
+struct RingBufs {
+  double* x;
+  double* g;
+  const double* e;
+  const int64_t* off;
+  const int* n;
+  int D;
+  hipStream_t s;
+};
+RingBufs ring_bufs(rp_ring* r);
 }  // namespace rp

@@ -453,6 +453,9 @@ const std::vector<int>& ring_n(const rp_ring* r) { return r->n; }
 const std::vector<int64_t>& ring_off(const rp_ring* r) { return r->off; }
 int ring_D(const rp_ring* r) { return r->D; }
 int ring_upload_x(rp_ring* r, const double* x) { return r->d_x.upload(x, r->tot, r->s); }
+RingBufs ring_bufs(rp_ring* r) {
+  return RingBufs{r->d_x.p, r->d_g.p, r->d_e.p, r->d_off.p, r->d_n.p, r->D, r->s};
+}
 int ring_download(rp_ring* r, double* e, double* g) {
   int rc;
   if (e && (rc = r->d_e.download(e, r->nchain, r->s))) return rc;

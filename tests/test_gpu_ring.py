@@ -157,6 +157,33 @@ def test_instanton_converges_to_scipy_optimum():
         assert abs(ld[k] - O.lndetj(ev)) <= 1e-9 * abs(O.lndetj(ev))
 
 
+def test_instanton_batch_is_per_chain_and_caps():
+    """Every chain of a batch follows its own search: chains of at most one bead tile give
+    bit-identical optima alone and in a batch; maxiter caps the iterations (status 2); a
+    start that already meets pgtol returns at once (status 0, no iteration)."""
+    sp = specs.evb()
+    pes = Pes(sp)
+    ns = [16, 40, 12]
+    betan = [3000.0 / n for n in ns]
+    x0 = [_linear_path(sp, n) for n in ns]
+    ring = Ring(pes, sp.mass, ns, betan, sp.well1, sp.well2)
+    xo, act, it, st = ring.instanton(x0, pgtol=1e-8, factr=10.0)
+    assert np.all(st == 0), st
+    for k, n in enumerate(ns):
+        one = Ring(pes, sp.mass, [n], [betan[k]], sp.well1, sp.well2)
+        x1, a1, i1, s1 = one.instanton([x0[k]], pgtol=1e-8, factr=10.0)
+        assert np.array_equal(x1[0], xo[k]) and a1[0] == act[k] and i1[0] == it[k]
+        # the optimum stays on the device: U_M there is the returned action
+        assert one.UM()[0] == pytest.approx(a1[0], rel=1e-14)
+        one.close()
+    xc, ac, ic, sc = ring.instanton(x0, maxiter=3)
+    assert np.all(sc == 2) and np.all(ic == 3)
+    assert np.all(ac < ring.UM(x0))
+    xs, as_, is_, ss = ring.instanton(xo, pgtol=1.0)
+    assert np.all(ss == 0) and np.all(is_ == 0)
+    assert all(np.array_equal(a, b) for a, b in zip(xs, xo))
+
+
 def test_water_param_grad_ring():
     sp = specs.water_cluster(nmol=2)
     pes = Pes(sp)
