@@ -174,6 +174,11 @@ int rp_batch_families(rp_batch *b, int *nfam, int *n, int *count);
  * launch timings, then the steps-per-launch candidates; it keeps the fastest and reports
  * the steps and the majority wave count. */
 int rp_batch_set_launch(rp_batch *b, int steps_per_launch, int waves);
+/* rp_batch_run replays each family's block of steps_per_launch steps as a captured HIP
+ * graph, between that family's timing events, when it draws from the Philox streams
+ * (enable = 1, the default); enable = 0 launches the kernels directly.  Injected-noise runs
+ * always launch directly.  The results are the same either way. */
+int rp_batch_set_graphs(rp_batch *b, int enable);
 int rp_batch_autotune(rp_batch *b, int trial_steps, int *best_steps, int *best_waves);
 /* hipEvent-timed average duration of one fused-kernel launch per family in the last
  * rp_batch_run: avg_ms[nfam]; launches = launches per family */

@@ -40,6 +40,7 @@ struct BatchArgs {
   int first;          // first polymer (batch order) of this launch's family
   int nsteps;
   int64_t step0, imin;
+  const int64_t* step0p;        // step counter in device memory (graph replays), else nullptr
   double dt, limit, stdev0;
   double c, sq, sp, c1sq, kappa;  // uniform rotation / O-step scalars (see above)
   double tau;

@@ -50,7 +50,32 @@ struct rp_batch {
   std::vector<hipEvent_t> fev;  // per family: start, end
   std::vector<double> fam_ms;   // per family: summed launch time of the last run
   int64_t last_launches = 0;
+  // each family's launch of a step block (the last family's followed by the step counter
+  // update) captured as a HIP graph and replayed between its timing events; a graph is
+  // rebuilt when anything it captured changes
+  int use_graph = 1;
+  struct GraphKey {
+    int fam = -1;
+    int steps = -1;
+    int64_t imin = -1, kick_cap = -1;
+    const void* kicks = nullptr;
+    const void* pathd = nullptr;
+    int npath = -1;
+    std::vector<int> waves;
+    bool operator==(const GraphKey& o) const {
+      return fam == o.fam && steps == o.steps && imin == o.imin && kick_cap == o.kick_cap &&
+             kicks == o.kicks &&
+             pathd == o.pathd && npath == o.npath && waves == o.waves;
+    }
+  };
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;  // recent family step blocks
+  DevBuf<int64_t> d_step0;
 };
+
+static void graph_drop(rp_batch* b) {
+  for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
+  b->graphs.clear();
+}
 
 // Folded halves of T for even N (H = N/2, HPAD = H rounded up to 16): row r of the odd
 // half is mode 2r+1 and row r of the even half mode 2r+2 (1-based), both over beads
@@ -306,6 +331,8 @@ extern "C" int rp_batch_create(rp_pes* pes, const double* mass, int npoly,
 extern "C" int rp_batch_destroy(rp_batch* b) {
   if (!b) return RP_OK;
   if (b->s) (void)hipStreamSynchronize(b->s);
+  graph_drop(b);
+  b->d_step0.free();
   for (Family& f : b->fam) {
     f.d_T.free();
     f.d_Tf.free();
@@ -426,12 +453,86 @@ static int run_init(rp_batch* b, const double* dn, int64_t cap) {
   return RP_OK;
 }
 
+__global__ void batch_advance_kernel(int64_t* step0, int steps) {
+  if (threadIdx.x == 0) *step0 += steps;
+}
+
+static rp_batch::GraphKey graph_key(const rp_batch* b, size_t fi, int steps, int64_t imin) {
+  rp_batch::GraphKey k;
+  k.fam = (int)fi;
+  k.steps = steps;
+  k.imin = imin;
+  k.kick_cap = b->kick_cap;
+  k.kicks = b->d_kicks.p;
+  k.pathd = b->d_pathd.p;
+  k.npath = b->npath;
+  for (const Family& f : b->fam) k.waves.push_back(f.waves);
+  return k;
+}
+
+// Capture family fi's launch of one step block (and, for the last family, the device step
+// counter advance).  The stream leaves capture mode on every path.
+static int graph_build(rp_batch* b, size_t fi, int steps, int64_t imin, hipGraphExec_t* out) {
+  int rc;
+  if (!b->d_step0.p && (rc = b->d_step0.alloc(1))) return rc;
+  Family& f = b->fam[fi];
+  BatchArgs a = base_args(b, f, 0.5 * b->dt);
+  a.nsteps = steps;
+  a.step0 = 0;
+  a.step0p = b->d_step0.p;
+  a.imin = imin;
+  a.kicks = b->d_kicks.p;
+  a.kick_cap = b->kick_cap;
+  RP_HIP(hipStreamBeginCapture(b->s, hipStreamCaptureModeRelaxed));
+  rc = launch_family(b, a, f.count);
+  if (rc == RP_OK && fi + 1 == b->fam.size()) {
+    batch_advance_kernel<<<1, 64, 0, b->s>>>(b->d_step0.p, steps);
+    if (hipGetLastError() != hipSuccess)
+      rc = set_error(RP_ERR_HIP, "batch graph: capture of the step counter failed");
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->s, &g);
+  if (rc == RP_OK && e != hipSuccess)
+    rc = set_error(RP_ERR_HIP, "batch graph: capture failed (%s)", hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  if (rc == RP_OK) {
+    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess)
+      rc = set_error(RP_ERR_HIP, "batch graph: instantiate failed (%s)", hipGetErrorString(ei));
+  }
+  if (g) (void)hipGraphDestroy(g);
+  if (rc != RP_OK) return rc;
+  if (b->graphs.size() >= 64) {
+    (void)hipGraphExecDestroy(b->graphs.front().second);
+    b->graphs.erase(b->graphs.begin());
+  }
+  b->graphs.emplace_back(graph_key(b, fi, steps, imin), exec);
+  *out = exec;
+  return RP_OK;
+}
+
 // One launch per family of `steps` steps, each bracketed by its own events; per-family
-// durations are accumulated into fam_ms.
+// durations are accumulated into fam_ms.  With the engine's own streams (no injected draws)
+// the block is a replay of its captured graph.
 static int run_block(rp_batch* b, int steps, int64_t imin, const double* dn, int64_t cap,
                      std::vector<double>& fam_ms) {
   int rc;
-  for (size_t fi = 0; fi < b->fam.size(); ++fi) {
+  if (b->use_graph && !dn) {
+    std::vector<hipGraphExec_t> exec(b->fam.size(), nullptr);
+    for (size_t fi = 0; fi < b->fam.size(); ++fi) {
+      const rp_batch::GraphKey key = graph_key(b, fi, steps, imin);
+      for (auto& g : b->graphs)
+        if (g.first == key) exec[fi] = g.second;
+      if (!exec[fi] && (rc = graph_build(b, fi, steps, imin, &exec[fi]))) return rc;
+    }
+    RP_HIP(hipMemcpyAsync(b->d_step0.p, &b->steps_done, sizeof(int64_t), hipMemcpyHostToDevice, b->s));
+    for (size_t fi = 0; fi < b->fam.size(); ++fi) {
+      RP_HIP(hipEventRecord(b->fev[2 * fi], b->s));
+      RP_HIP(hipGraphLaunch(exec[fi], b->s));
+      RP_HIP(hipEventRecord(b->fev[2 * fi + 1], b->s));
+    }
+  } else {
+    for (size_t fi = 0; fi < b->fam.size(); ++fi) {
     Family& f = b->fam[fi];
     BatchArgs a = base_args(b, f, 0.5 * b->dt);
     a.nsteps = steps;
@@ -444,6 +545,7 @@ static int run_block(rp_batch* b, int steps, int64_t imin, const double* dn, int
     RP_HIP(hipEventRecord(b->fev[2 * fi], b->s));
     if ((rc = launch_family(b, a, f.count))) return rc;
     RP_HIP(hipEventRecord(b->fev[2 * fi + 1], b->s));
+    }
   }
   RP_HIP(hipEventSynchronize(b->fev[2 * b->fam.size() - 1]));
   fam_ms.resize(b->fam.size(), 0.0);
@@ -612,6 +714,13 @@ extern "C" int rp_batch_set_cursor(rp_batch* b, const int64_t* cursor, const int
   }
   if ((rc = b->d_pi64.upload(pi.data(), pi.size(), b->s))) return rc;
   RP_HIP(hipStreamSynchronize(b->s));
+  return RP_OK;
+}
+
+extern "C" int rp_batch_set_graphs(rp_batch* b, int enable) {
+  RP_REQUIRE(b, "rp_batch_set_graphs: NULL batch");
+  b->use_graph = enable ? 1 : 0;
+  if (!enable) graph_drop(b);
   return RP_OK;
 }
 

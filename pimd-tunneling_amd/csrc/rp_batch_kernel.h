@@ -539,8 +539,9 @@ __global__ void __launch_bounds__(W * 64, (W == 4 && UPW == 1) ? 2 : 1)
   const int64_t need = (int64_t)N * D;  // draws per O-step / resample / re-initialisation
   __syncthreads();
   bool stop = false;
+  const int64_t step0 = A.step0p ? *A.step0p : A.step0;
   for (int s = 0; s < A.nsteps; ++s) {
-    const int64_t ii = A.step0 + s + 1;  // the reference's step counter
+    const int64_t ii = step0 + s + 1;  // the reference's step counter
     int ph[6], nph = 0;
     bool resample = false;
     if (langevin) {

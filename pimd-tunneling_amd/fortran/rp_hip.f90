@@ -282,6 +282,11 @@ module rp_hip
        type(c_ptr), value :: b
        integer(c_int), value :: steps_per_launch, waves
      end function rp_batch_set_launch
+     integer(c_int) function rp_batch_set_graphs(b, enable) bind(C, name="rp_batch_set_graphs")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: b
+       integer(c_int), value :: enable
+     end function rp_batch_set_graphs
      integer(c_int) function rp_batch_autotune(b, trial_steps, best_steps, best_waves) &
           bind(C, name="rp_batch_autotune")
        import :: c_int, c_ptr

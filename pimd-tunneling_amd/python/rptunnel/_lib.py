@@ -84,6 +84,7 @@ _SIGS = {
     "rp_batch_families": [vp, ip, ip, ip],
     "rp_batch_ti_readout": [vp, dp, i64p, i64p],
     "rp_batch_set_launch": [vp, C.c_int, C.c_int],
+    "rp_batch_set_graphs": [vp, C.c_int],
     "rp_batch_autotune": [vp, C.c_int, ip, ip],
     "rp_batch_last_kernel_ms": [vp, dp, i64p],
     "rp_ring_create": [vp, dp, C.c_int, ip, dp, dp, dp, C.c_int, C.POINTER(vp)],

@@ -119,6 +119,10 @@ class Batch:
     def set_launch(self, steps_per_launch=0, waves=0):
         L.check(L.lib().rp_batch_set_launch(self.h, steps_per_launch, waves))
 
+    def set_graphs(self, enable=True):
+        """Replay step blocks as captured HIP graphs (default) or launch kernels directly."""
+        L.check(L.lib().rp_batch_set_graphs(self.h, int(bool(enable))))
+
     def autotune(self, trial_steps=64):
         bs, bw = C.c_int(0), C.c_int(0)
         L.check(L.lib().rp_batch_autotune(self.h, trial_steps, C.byref(bs), C.byref(bw)))

@@ -97,6 +97,40 @@ def test_batch_pile_philox_and_blocks():
             assert rel(xg, xo) <= 1e-10 and rel(pg, po) <= 1e-10
 
 
+@pytest.mark.parametrize("thermostat", [2, 1])
+def test_graph_replays_match_direct_launches(thermostat):
+    """Step blocks replayed from captured HIP graphs give bit for bit the states, TI sums,
+    counts and stream positions of direct launches, over several runs of different lengths
+    (a remainder block, a changed imin) with host reads between the replays."""
+    sp = specs.evb()
+    pes = Pes(sp)
+    pols = _polymers(sp, [64, 32, 96, 64], xis=[0.2, 0.5, 0.7, 0.9])
+    bs = []
+    for graphs in (True, False):
+        b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, thermostat=thermostat, noutput=5, seed=21)
+        b.set_graphs(graphs)
+        b.set_launch(steps_per_launch=8)
+        for k, pol in enumerate(pols):
+            x, p = _state(sp, pol["n"], 40 + k)
+            b.set_state(k, x, p)
+        bs.append(b)
+    done = 0
+    for nsteps, imin in ((17, 3), (16, 0), (9, 0), (8, 0)):
+        for b in bs:
+            b.run(nsteps, imin=imin)
+        done += nsteps
+        for k in range(len(pols)):
+            xg, pg = bs[0].get_state(k)
+            xe, pe = bs[1].get_state(k)
+            assert np.array_equal(xg, xe) and np.array_equal(pg, pe), (nsteps, k)
+        sg, cg, _ = bs[0].ti_readout()
+        se, ce, _ = bs[1].ti_readout()
+        assert np.array_equal(sg, se) and np.array_equal(cg, ce)
+        assert np.all(cg == done - 3)  # the step counter carries over between replays
+        assert np.array_equal(bs[0].cursors(), bs[1].cursors())
+        assert np.array_equal(bs[0].kick_cursors(), bs[1].kick_cursors())
+
+
 def test_batch_init_path_and_reinit():
     sp = specs.evb()
     pes = Pes(sp)
