@@ -145,10 +145,11 @@ static int batch_flags(rp_batch* b) {
 
 // Waves per workgroup by default: 8 (two per SIMD, which hides the latency of the
 // elementwise phases) when the MFMA units spread over 8 waves at least as evenly as over
-// 4 and there are at least 6 of them and at most 16; 4 otherwise.
+// 4 and there are at least 8 of them and at most 16; 4 otherwise (at 6 units, two 4-wave
+// workgroups of two units per wave share a CU and beat one 8-wave workgroup).
 static int default_waves(int DT, int NPAD) {
   const int units = (NPAD >> 5) * DT;
-  if (units < 6 || units > 16) return 4;
+  if (units < 8 || units > 16) return 4;
   const double even8 = units / (8.0 * ((units + 7) / 8)), even4 = units / (4.0 * ((units + 3) / 4));
   return even8 >= even4 ? 8 : 4;
 }
