@@ -144,14 +144,13 @@ static int batch_flags(rp_batch* b) {
 }
 
 // Waves per workgroup by default: 8 (two per SIMD, which hides the latency of the
-// elementwise phases) when the MFMA units spread over 8 waves at least as evenly as over
-// 4 and there are at least 8 of them and at most 16; 4 otherwise (at 6 units, two 4-wave
-// workgroups of two units per wave share a CU and beat one 8-wave workgroup).
+// elementwise phases) from 8 to 16 MFMA units, 4 otherwise.  Measured on the 27-dof EVB
+// (ms per 8-step launch of 5632 polymers, 4 / 8 waves): 6 units 3.52 / 3.86 (two 4-wave
+// workgroups of two units per wave share a CU), 8 units 5.50 / 4.45, 12 units 9.09 / 8.63,
+// 16 units 13.5 / 11.5.
 static int default_waves(int DT, int NPAD) {
   const int units = (NPAD >> 5) * DT;
-  if (units < 8 || units > 16) return 4;
-  const double even8 = units / (8.0 * ((units + 7) / 8)), even4 = units / (4.0 * ((units + 3) / 4));
-  return even8 >= even4 ? 8 : 4;
+  return (units >= 8 && units <= 16) ? 8 : 4;
 }
 
 static BatchArgs base_args(rp_batch* b, const Family& f, double h) {
