@@ -83,7 +83,10 @@ __global__ void __launch_bounds__(kRingTile) ring_um_kernel(RingArgs A, const do
 #pragma unroll
     for (int d = 0; d < MAXD; ++d) xs[d] = (d < D) ? xc[(size_t)d * n + i] : 0.0;
     e += eval_kind<KIND, MAXD>(P, xs, gs);
-    for (int d = 0; d < D; ++d) {
+    // compile-time dof indices keep xs / gs in registers
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+      if (d >= D) continue;
       const double m = A.massd[d], xi = xs[d];
       const double* xd = xc + (size_t)d * n;
       if (i < n - 1) {
@@ -132,7 +135,9 @@ __global__ void __launch_bounds__(kRingTile) ring_hvp_kernel(RingArgs A, const d
   for (int d = 0; d < MAXD; ++d)
     xs[d] = (d < D) ? Dual(x[o + (size_t)d * n + i], v[o + (size_t)d * n + i]) : Dual(0.0);
   (void)eval_kind<KIND, MAXD>(P, xs, gs);
-  for (int d = 0; d < D; ++d) {
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d) {
+    if (d >= D) continue;
     const double m = A.massd[d];
     const double* vd = v + o + (size_t)d * n;
     double s;
@@ -149,8 +154,9 @@ __global__ void __launch_bounds__(kRingTile) ring_hvp_kernel(RingArgs A, const d
 // Potential Hessian blocks of every bead from D forward-over-reverse sweeps:
 // blocks[gbead][i][k] = d grad_k / d x_i.  Thread (bead, i).
 template <int KIND, int MAXD>
-__global__ void ring_hblock_kernel(RingArgs A, const double* __restrict__ x, int single_first,
-                                   double* __restrict__ blocks) {
+__global__ void __launch_bounds__(128) ring_hblock_kernel(RingArgs A, const double* __restrict__ x,
+                                                          int single_first,
+                                                          double* __restrict__ blocks) {
   __shared__ double pl[StagedPes<KIND, MAXD>::kLds];
   const int c = blockIdx.y;
   const int n = A.n[c], D = A.D;
@@ -167,7 +173,9 @@ __global__ void ring_hblock_kernel(RingArgs A, const double* __restrict__ x, int
     xs[d] = (d < D) ? Dual(x[o + (size_t)d * n + src], d == i ? 1.0 : 0.0) : Dual(0.0);
   (void)eval_kind<KIND, MAXD>(P, xs, gs);
   double* blk = blocks + ((size_t)(A.boff[c] + bead) * D + i) * D;
-  for (int k = 0; k < D; ++k) blk[k] = gs[k].d;
+#pragma unroll
+  for (int k = 0; k < MAXD; ++k)
+    if (k < D) blk[k] = gs[k].d;
 }
 
 // UMhessian's band (instantonmod.f90:155-217) from the blocks of one chain, in the
