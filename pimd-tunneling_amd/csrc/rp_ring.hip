@@ -240,83 +240,132 @@ __global__ void __launch_bounds__(kRingThreads) ring_param_grad_kernel(RingArgs 
 namespace {
 constexpr int kMaxD = 56;  // ndof bound of the log-determinant workspace (LDS)
 
-constexpr int kPer = (kMaxD * kMaxD + kRingThreads - 1) / kRingThreads;  // elements per thread
-
-// In-place Gauss-Jordan inverse of the D x D matrix in M (LDS, row stride D), without
-// pivoting, every element updated in parallel at each of the D eliminations.  The pivot of
-// elimination k is the k-th pivot of the matrix's LDL^T factorisation (a ratio of leading
-// principal minors), so their logs sum to log|det| and their signs give the inertia
-// (Sylvester).  lg / neg are valid in thread 0.
-__device__ void gj_inverse(int D, double* M, double& lg, int& neg) {
-  const int tid = threadIdx.x, nt = blockDim.x, DD = D * D;
-  double l = 0.0;
-  int ng = 0;
-  for (int k = 0; k < D; ++k) {
-    const double p = M[k * D + k], ip = 1.0 / p;
-    double nv[kPer];
-    int cnt = 0;
-    for (int e = tid; e < DD; e += nt, ++cnt) {
-      const int r = e / D, cc = e - r * D;
-      const double mrk = M[r * D + k], mkc = M[k * D + cc];
-      double v;
-      if (r == k)
-        v = (cc == k) ? ip : mkc * ip;
-      else if (cc == k)
-        v = -mrk * ip;
-      else
-        v = M[e] - mrk * mkc * ip;
-      nv[cnt] = v;
-    }
-    l += log(fabs(p));
-    ng += p < 0.0;
-    __syncthreads();
-    cnt = 0;
-    for (int e = tid; e < DD; e += nt, ++cnt) M[e] = nv[cnt];
-    __syncthreads();
+// One wave per chain.  Lane l owns the elements e = l + 64 q (q < P) of every D x D block, so
+// the Hessian block of the next bead is loaded into registers while the current one is
+// eliminated, and the Schur update uses the previous inverse where it sits in LDS.  The
+// Gauss-Jordan inverse runs without pivoting, double-buffered in LDS (one barrier per
+// elimination): the pivot of elimination k is the k-th pivot of the matrix's LDL^T
+// factorisation (a ratio of leading principal minors), so their logs sum to log|det| and
+// their signs give the inertia (Sylvester).
+// Barrier of ring_logdet_kernel: LDS ordering only, so the global loads of the next block
+// stay in flight across it (a one-wave workgroup needs no hardware barrier at all: its LDS
+// operations complete in issue order).
+template <int T>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (T == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
   }
-  lg = l;
-  neg = ng;
 }
 
-__global__ void __launch_bounds__(kRingThreads) ring_logdet_kernel(
+// T threads per chain.  Thread t owns the elements e = t + T q (q < P) of every D x D block,
+// so the Hessian block of the next bead is loaded into registers while the current one is
+// eliminated, and the Schur update uses the previous inverse where it sits in LDS.  The
+// Gauss-Jordan inverse runs without pivoting, double-buffered in LDS (one barrier per
+// elimination): the pivot of elimination k is the k-th pivot of the matrix's LDL^T
+// factorisation (a ratio of leading principal minors), so their logs sum to log|det| and
+// their signs give the inertia (Sylvester).
+template <int P, int T>
+__global__ void __launch_bounds__(T) ring_logdet_kernel(
     int D, const int* __restrict__ nn, const int64_t* __restrict__ boff,
     const double* __restrict__ betan, const double* __restrict__ massd,
     const double* __restrict__ blocks, const double* __restrict__ shift, double* __restrict__ sinv,
     double* __restrict__ logdet, int* __restrict__ nneg) {
-  __shared__ double Sm[kMaxD * kMaxD];
-  const int c = blockIdx.x, n = nn[c], tid = threadIdx.x;
+  // every LDS element t + T q exists, and so does every r D + k <= t + T q + D - 1 read for
+  // it; the elements beyond D*D hold values nobody reads back, which keeps every load and
+  // store unconditional (no branch, so the loads of the next block stay in flight)
+  constexpr int kTile = T * P + kMaxD > kMaxD * (kMaxD + 1) ? T * P + kMaxD : kMaxD * (kMaxD + 1);
+  __shared__ double Sb[2][kTile];
+  const int c = blockIdx.x, n = nn[c], tid = threadIdx.x, DD = D * D;
   const double bn2 = betan[c] * betan[c], c2 = 1.0 / (bn2 * bn2);
   const double sig = shift ? shift[c] : 0.0;  // factor H - sig I
+  int er[P], ec[P], eg[P];
+  double wgt[P], diag[P], nxt[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    const int e = tid + T * q;
+    eg[q] = min(e, DD - 1);  // clamped index of the global block loads
+    er[q] = e / D;
+    ec[q] = e - er[q] * D;
+    const int rm = min(er[q], D - 1);
+    wgt[q] = 1.0 / sqrt(massd[rm] * massd[ec[q]]);
+    diag[q] = er[q] == ec[q] ? 2.0 / bn2 - sig : 0.0;
+  }
+  const double* base = blocks + (size_t)boff[c] * DD;
+#pragma unroll
+  for (int q = 0; q < P; ++q) nxt[q] = base[eg[q]];
   double lg_tot = 0.0;
-  int neg_tot = 0;
+  int neg_tot = 0, cur = 0;
   for (int i = 0; i < n; ++i) {
-    const double* blk = blocks + (size_t)(boff[c] + i) * D * D;
-    const double* prev = sinv + (size_t)(boff[c] + i - 1) * D * D;
-    const bool coupled = i >= 2;  // pair (i-1, i) with i-1 >= 1
-    for (int e = tid; e < D * D; e += blockDim.x) {
-      const int r = e / D, cc = e - r * D;
-      // A_i(r, cc) = 2/bn^2 delta + H(cc -> r) / sqrt(m_r m_cc); blocks are d grad_k / d x_i
-      double v = blk[r * D + cc] / sqrt(massd[r] * massd[cc]);  // lower triangle convention
-      if (r == cc) v += 2.0 / bn2 - sig;
-      if (coupled) v -= c2 * prev[e];
-      Sm[e] = v;
+    // A_i(r, cc) = 2/bn^2 delta + H(cc -> r) / sqrt(m_r m_cc), minus c2 S_{i-1}^-1 when the
+    // pair (i-1, i) is coupled (i >= 2; bead 0 is not coupled to bead 1)
+    const bool coupled = i >= 2;
+    double* M = Sb[cur];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const int e = tid + T * q;
+      const double v = nxt[q] * wgt[q] + diag[q];
+      const double prev = M[e];
+      M[e] = coupled ? v - c2 * prev : v;
     }
-    __syncthreads();
-    double lg;
-    int ng;
-    gj_inverse(D, Sm, lg, ng);
-    if (tid == 0) {
-      lg_tot += lg;
+    // the next bead's block (a valid address for the last bead too), in flight during the
+    // elimination
+    const double* nb = base + (size_t)min(i + 1, n - 1) * DD;
+#pragma unroll
+    for (int q = 0; q < P; ++q) nxt[q] = nb[eg[q]];
+    lds_sync<T>();
+    double mypiv = 1.0;
+    for (int k = 0; k < D; ++k) {
+      const double* A = Sb[cur];
+      double* B = Sb[cur ^ 1];
+      const double p = A[k * D + k], ip = 1.0 / p;
+      mypiv = tid == k ? p : mypiv;
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int e = tid + T * q, r = er[q], cc = ec[q];
+        const double mrk = A[r * D + k], mkc = A[k * D + cc], a = A[e];
+        const double vr = cc == k ? ip : mkc * ip;
+        const double vo = cc == k ? -mrk * ip : a - mrk * mkc * ip;
+        B[e] = r == k ? vr : vo;
+      }
+      cur ^= 1;
+      lds_sync<T>();
+    }
+    // pivots: log|p| and the sign count; threads k < D <= 56 of wave 0 hold pivot k
+    if (tid < 64) {
+      double l = tid < D ? log(fabs(mypiv)) : 0.0;
+      int ng = (tid < D && mypiv < 0.0) ? 1 : 0;
+      for (int o = 32; o > 0; o >>= 1) {
+        l += __shfl_xor(l, o);
+        ng += __shfl_xor(ng, o);
+      }
+      lg_tot += l;
       neg_tot += ng;
     }
-    double* out = sinv + (size_t)(boff[c] + i) * D * D;
-    for (int e = tid; e < D * D; e += blockDim.x) out[e] = Sm[e];
-    __syncthreads();
+    double* out = sinv + (size_t)(boff[c] + i) * DD;
+    const double* Minv = Sb[cur];
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+      if (tid + T * q < DD) out[tid + T * q] = Minv[tid + T * q];
   }
   if (tid == 0) {
     logdet[c] = lg_tot;
     nneg[c] = neg_tot;
   }
+}
+
+// threads and per-thread element count of ring_logdet_kernel for D (D*D <= T P)
+template <typename F>
+int with_logdet_kernel(int D, F&& f) {
+  if (D * D <= 64) return f(ring_logdet_kernel<1, 64>, 64);
+  if (D * D <= 256) return f(ring_logdet_kernel<1, 256>, 256);
+  if (D * D <= 768) return f(ring_logdet_kernel<3, 256>, 256);
+  if (D * D <= 1536) return f(ring_logdet_kernel<6, 256>, 256);
+  return f(ring_logdet_kernel<13, 256>, 256);
 }
 
 // Lanes per row of the D x D mat-vecs below: a power of two, at most 64 (one wave), with
@@ -640,10 +689,14 @@ int ring_lowest_eigen(rp_ring* r, double* eta) {
     // shift just off the estimate so the shifted matrix stays factorable
     for (double& v : sh) v = v * (1.0 - 1e-9);
     if ((rc = d_shift.upload(sh.data(), r->nchain, r->s))) return rc;
-    ring_logdet_kernel<<<r->nchain, kRingThreads, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p,
-                                                             r->d_massd.p, r->d_blocks.p, d_shift.p,
-                                                             r->d_sinv.p, r->d_ld.p, r->d_nneg.p);
-    RP_HIP(hipGetLastError());
+    if ((rc = with_logdet_kernel(r->D, [&](auto kern, int nt) -> int {
+          kern<<<r->nchain, nt, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p, r->d_massd.p,
+                                          r->d_blocks.p, d_shift.p, r->d_sinv.p, r->d_ld.p,
+                                          r->d_nneg.p);
+          RP_HIP(hipGetLastError());
+          return RP_OK;
+        })))
+      return rc;
     ring_eta1_kernel<<<r->nchain, kRingThreads, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p,
                                                            r->d_sinv.p, r->d_work.p, d_shift.p, 4, 0,
                                                            r->d_eta.p);
@@ -666,10 +719,13 @@ extern "C" int rp_ring_logdet(rp_ring* r, int singlewell, int skip_lowest, doubl
   if (r->d_sinv.n < nb && (rc = r->d_sinv.alloc(nb))) return rc;
   if ((rc = r->d_ld.alloc(r->nchain)) || (rc = r->d_nneg.alloc(r->nchain)) || (rc = r->d_eta.alloc(r->nchain)))
     return rc;
-  ring_logdet_kernel<<<r->nchain, kRingThreads, 0, r->s>>>(D, r->d_n.p, r->d_boff.p, r->d_betan.p,
-                                                           r->d_massd.p, r->d_blocks.p, nullptr,
-                                                           r->d_sinv.p, r->d_ld.p, r->d_nneg.p);
-  RP_HIP(hipGetLastError());
+  if ((rc = with_logdet_kernel(D, [&](auto kern, int nt) -> int {
+        kern<<<r->nchain, nt, 0, r->s>>>(D, r->d_n.p, r->d_boff.p, r->d_betan.p, r->d_massd.p,
+                                        r->d_blocks.p, nullptr, r->d_sinv.p, r->d_ld.p, r->d_nneg.p);
+        RP_HIP(hipGetLastError());
+        return RP_OK;
+      })))
+    return rc;
   std::vector<double> ld(r->nchain), et(r->nchain, 0.0);
   std::vector<int> ng(r->nchain);
   if ((rc = r->d_ld.download(ld.data(), r->nchain, r->s)) ||
