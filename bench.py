@@ -195,6 +195,66 @@ def instanton_sample(pes, sp, nchain=256, reps=10, maxiter=300, seed=4):
                       "pgtol": 1e-5, "factr": 1e6, "iteration_cap": maxiter}}
 
 
+def water_sample(steps=16, warmup=8, ntraj=1024, n=128, nchain=256):
+    """The 6-water flexible cluster (18 atoms, 54 dof): batched PIMD-TI at N = 128 with
+    ntraj trajectories x 11 lambda nodes (Langevin, the same step parameters as the
+    headline), and the adjoint sensitivities dU_M/d(qH, De) of nchain instanton chains of
+    N beads.  Timed on the device with the inputs resident; bead-steps and bead-evaluations
+    per second."""
+    from rptunnel import Batch, Pes, Ring, specs
+
+    from rptunnel import ti
+
+    sp = specs.water_cluster(nmol=6)
+    pes = Pes(sp)
+    # the path turns molecule 0's H pair about its bisector (a straight line to the
+    # exchanged well would run the two H atoms through each other)
+    path = np.ascontiguousarray(sp.extra["path"].T)
+    lampath = specs.arc_lampath(path)
+    xi, _ = gauleg(NLAMBDA)
+    ends, grads = ti._spline_nodes(lampath, path, xi)
+    pols = [dict(n=n, betan=BETA / (n + 1), xi=float(x), a=path[:, 0], b=ends[il],
+                 dbdl=grads[il], stream=il * ntraj + r)
+            for il, x in enumerate(xi) for r in range(ntraj)]
+    b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, seed=7)
+    b.set_path(lampath, path)
+    b.init_path()
+    b.set_launch(8)
+    b.run(warmup)
+    rp_sync()
+    t0 = time.perf_counter()
+    b.run(steps)
+    rp_sync()
+    wall = time.perf_counter() - t0
+    ms, launches = b.last_kernel_ms()
+    sums, _, _ = b.ti_readout()
+    assert np.all(np.isfinite(sums)), "non-finite TI sums (water)"
+    b.close()
+    k = np.arange(1, n + 1)
+    chain = (np.outer(sp.well1, n - k) + np.outer(sp.well2, k - 1)) / (n - 1)
+    ring = Ring(pes, sp.mass, [n] * nchain, [BETA / n] * nchain, sp.well1, sp.well2)
+    ring.set_x([chain] * nchain)
+    ring.param_grad()
+    t1 = time.perf_counter()
+    for _ in range(5):
+        pg = ring.param_grad()
+    tpg = (time.perf_counter() - t1) / 5
+    assert np.all(np.isfinite(pg))
+    ring.close()
+    return {"workload": f"water6 (54 dof) batched TI: N = {n}, {ntraj} traj x {NLAMBDA} lambda "
+                        f"({len(pols)} polymers), Langevin; dU_M/d(qH, De) of {nchain} chains "
+                        f"of N = {n}",
+            "bead_steps_per_s": len(pols) * n * steps / wall,
+            "ms_per_step": 1e3 * wall / steps,
+            "avg_launch_ms": round(float(ms[0]), 4), "steps_per_launch": 8,
+            "param_grad_bead_evals_per_s": nchain * n / tpg}
+
+
+def rp_sync():
+    import rptunnel
+    rptunnel.lib().rp_finalize()  # device synchronize
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +266,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-instanton", action="store_true",
                     help="skip the batched-instanton sample reported beside the headline")
+    ap.add_argument("--no-water", action="store_true",
+                    help="skip the water-cluster sample reported beside the headline")
     ap.add_argument("--steps-per-launch", type=int, default=8,
                     help="steps fused into one kernel launch (warm-up and timed steps are "
                          "multiples of it, so every launch of a family has the same shape)")
@@ -284,6 +346,11 @@ def main():
         log("instanton sample (256 chains)")
         inst = instanton_sample(pes, sp)
 
+    water = None
+    if not args.no_water:
+        log("water-cluster sample (N = 128, 11264 polymers)")
+        water = water_sample()
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -323,6 +390,7 @@ def main():
                      "families": fam_rows},
         "cpu_baseline": cpu,
         "instanton": inst,
+        "water": water,
     }
     print(json.dumps(out), flush=True)
 
