@@ -337,10 +337,88 @@ __device__ __forceinline__ void nm_phase(const BatchArgs& A, double* S, const do
   });
 }
 
+// B for the water cluster with one (bead, atom) per thread: the atom's gradient from its own
+// molecule's Morse bonds and HOH bend and from every atom of the other molecules (charges,
+// and Lennard-Jones between oxygens), the positions read from LDS.  Each pair term is made
+// by both of its atoms (twice the pair arithmetic of one evaluation per bead), and a thread
+// holds one atom's three components instead of a bead's 3 x 18: nothing spills.  The terms
+// are those of water_eval (rp_pes.h).
+template <int NMOL, int DT, int W>
+__device__ __forceinline__ void water_kick_atoms(const BatchArgs& A, double* S, int tid, bool& nan) {
+  const double* dp = A.pes.dp;
+  const double De = dp[0], al = dp[1], re = dp[2], kb = dp[3], th0 = dp[4], eps = dp[5],
+               sig = dp[6], qH = dp[7];
+  constexpr int NA = 3 * NMOL;
+  const int N = A.N, LD = A.LD;
+  for (int it = tid; it < N * NA; it += W * 64) {
+    const int b = it / NA, i = it - b * NA;
+    const int m = i / 3, ai = i - 3 * m;  // molecule, and 0 = O, 1 / 2 = H
+    const double* xb = S + b * LD;
+    const double* xm = xb + 9 * m;
+    // own molecule: bonds u_h = H_h - O and the bend
+    double u[2][3], r[2], dEdr[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      double rr = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        u[h][d] = xm[3 * (h + 1) + d] - xm[d];
+        rr += u[h][d] * u[h][d];
+      }
+      r[h] = sqrt(rr);
+      const double e = exp(-al * (r[h] - re)), om = 1.0 - e;
+      dEdr[h] = (2.0 * De * al) * om * e / r[h];
+    }
+    const double dot = u[0][0] * u[1][0] + u[0][1] * u[1][1] + u[0][2] * u[1][2];
+    const double ir01 = 1.0 / (r[0] * r[1]);
+    const double cs = dot * ir01;
+    const double dth = acos(cs) - th0;
+    const double dEdc = -kb * dth / sqrt(1.0 - cs * cs);
+    const double i00 = cs / (r[0] * r[0]), i11 = cs / (r[1] * r[1]);
+    double g[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double t0 = dEdr[0] * u[0][d], t1 = dEdr[1] * u[1][d];
+      const double d0 = dEdc * (u[1][d] * ir01 - i00 * u[0][d]);
+      const double d1 = dEdc * (u[0][d] * ir01 - i11 * u[1][d]);
+      g[d] = ai == 0 ? -(t0 + t1) - (d0 + d1) : (ai == 1 ? t0 + d0 : t1 + d1);
+    }
+    // the other molecules
+    const double si = ai == 0 ? -2.0 : 1.0;
+    const double xi0 = xb[3 * i], xi1 = xb[3 * i + 1], xi2 = xb[3 * i + 2];
+    for (int j = 0; j < NA; ++j) {
+      if (j / 3 == m) continue;
+      const double dx0 = xi0 - xb[3 * j], dx1 = xi1 - xb[3 * j + 1], dx2 = xi2 - xb[3 * j + 2];
+      const double rr = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+      const double ir2 = 1.0 / rr, ir = sqrt(ir2);
+      const double sij = si * ((j % 3 == 0) ? -2.0 : 1.0);
+      double dEr = (-qH * qH * sij) * ir2 * ir;  // (dE/dr) / r
+      if (ai == 0 && j % 3 == 0) {
+        const double s2 = (sig * sig) * ir2, s6 = s2 * s2 * s2;
+        dEr += (4.0 * eps) * (-12.0 * s6 * s6 + 6.0 * s6) * ir2;
+      }
+      g[0] += dEr * dx0;
+      g[1] += dEr * dx1;
+      g[2] += dEr * dx2;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double* pp = S + b * LD + DT * 16 + 3 * i + d;
+      const double np = *pp - g[d] * A.dt;
+      nan |= (np != np);
+      *pp = np;
+    }
+  }
+}
+
 // B: p -= dt grad V(x) for every bead (verletmodule.f90:565-585)
 template <int KIND, int MAXD, int DT, int W>
 __device__ __forceinline__ void pes_kick(const BatchArgs& A, double* S, const double* PL, int tid,
                                          bool& nan) {
+  if constexpr (KIND == PES_WATER) {
+    water_kick_atoms<MAXD / 9, DT, W>(A, S, tid, nan);
+    return;
+  }
   const int N = A.N, D = A.D, LD = A.LD;
   for (int b = tid; b < N; b += W * 64) {
     double xs[MAXD], gs[MAXD];
