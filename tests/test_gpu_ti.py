@@ -81,7 +81,8 @@ def test_quartic_ti_pipeline_matches_oracle():
         pols.append(dict(n=n, betan=betan, xi=float(xi[i]), a=sp.well1, b=b,
                          dbdl=sp.well2 - sp.well1, stream=k))
         keep.append(k)
-    assert len(pols) == nint * nrep  # no node sits at b = 0 for this well pair
+    # the middle node of the symmetric well sits at b = 0: its replicas are skipped, integrand 0
+    assert len(pols) == (nint - 1) * nrep
     bt = Batch(pes, sp.mass, pols, tau, dt, gamma, seed=seed)
     bt.set_path(lampath, path)
     bt.init_path()
