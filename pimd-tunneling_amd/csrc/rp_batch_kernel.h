@@ -657,7 +657,8 @@ __global__ void __launch_bounds__(W * 64, (W == 4 && UPW == 1) ? 2 : 1)
       // copy in every phase: the per-lane LDS addresses are then recomputed where they are
       // used instead of being hoisted out of the step loop and spilled (measured: 16 % less
       // time per launch at N = 256, and under 256 VGPRs for two workgroups per CU at N = 96).
-      // One-unit waves keep the hoisted addresses, which fit.
+      // One-unit waves keep the hoisted addresses: the opaque copy removes their few spill
+      // slots (254 -> 237 VGPRs on the 27-dof EVB) but costs 4-5 % per launch at N = 32-128.
       int lane_k = lane;
       if constexpr (UPW >= 2) asm volatile("" : "+v"(lane_k));
       const int m = lane_k & 15, g = lane_k >> 4, tid = wave * 64 + lane_k;
