@@ -28,61 +28,93 @@ __device__ __forceinline__ double batch_draw(const BatchArgs& A, int64_t noff, u
   return philox_normal(A.seed, strm, idx);
 }
 
-// acc = T S for all units of this wave (see rp_batch.h for the unit layout).
-template <int DT, int UPW, int W>
-__device__ __forceinline__ void gemm_T(const double* __restrict__ T, int NPAD, const double* S,
-                                       int LD, int wave, int m, int g, d4 (&acc)[UPW][4]) {
-  const int units = (NPAD >> 5) * DT;
+// Number of unit slots of this wave that hold a real unit (wave-uniform): slots u < NU.
+template <int UPW, int W>
+__device__ __forceinline__ int active_units(int wave, int units) {
+  return max(0, min(UPW, (units - wave + W - 1) / W));
+}
+
+// f(integral_constant<int, nu>) for the wave-uniform nu in [0, UPW]: the GEMMs below are
+// instantiated per active-unit count, so a wave issues no MFMA for an empty slot.
+template <int UPW, typename F>
+__device__ __forceinline__ void with_units(int nu, F&& f) {
+  static_for<UPW + 1>([&](auto NU) __attribute__((always_inline)) {
+    if (nu == decltype(NU)::value) f(NU);
+  });
+}
+
+// acc = T S for the first NU units of this wave (see rp_batch.h for the unit layout);
+// accumulators of the other slots are zero and are dropped by the consumers.  Measured on
+// the 27-dof EVB at N = 192 (12 units in 16 slots of 8 waves): 8.64 -> 7.70 ms per 8-step
+// launch against recomputing the last unit in the empty slots.
+template <int DT, int UPW, int W, int NU>
+__device__ __forceinline__ void gemm_T_n(const double* __restrict__ T, int NPAD, const double* S,
+                                         int LD, int wave, int m, int g, d4 (&acc)[UPW][4]) {
 #pragma unroll
   for (int u = 0; u < UPW; ++u)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[u][c] = d4{0.0, 0.0, 0.0, 0.0};
-  const int nk = NPAD >> 2;  // a multiple of 8: NPAD is a multiple of 32
-  // Two register sets in ping-pong: the fragments of K-chunk kc+1 are loaded into one
-  // set while the MFMAs of chunk kc read the other, so a load has a whole chunk of
-  // MFMAs (16 x UPW x 4 cycles x 16) to arrive and no register copy sits between them.
-  // wave is wave-uniform (an SGPR), so the unit loop has no divergent control flow; a
-  // slot past the last unit recomputes the last unit and its result is dropped by the
-  // consumers, which keeps every MFMA unconditional.
-  int toff[UPW], soff[UPW];
+  if constexpr (NU > 0) {
+    const int nk = NPAD >> 2;  // a multiple of 8: NPAD is a multiple of 32
+    // Two register sets in ping-pong: the fragments of K-chunk kc+1 are loaded into one
+    // set while the MFMAs of chunk kc read the other, so a load has a whole chunk of
+    // MFMAs (16 x NU x 4 cycles x 16) to arrive and no register copy sits between them.
+    const int units = (NPAD >> 5) * DT;
+    int toff[NU], soff[NU];
 #pragma unroll
-  for (int u = 0; u < UPW; ++u) {
-    const int unit = min(wave + u * W, units - 1);
-    const int rp = unit / DT, dtile = unit - rp * DT;
-    toff[u] = g * NPAD + rp * 32 + m;
-    soff[u] = g * LD + dtile * 16 + m;
+    for (int u = 0; u < NU; ++u) {
+      const int unit = min(wave + u * W, units - 1);
+      const int rp = unit / DT, dtile = unit - rp * DT;
+      toff[u] = g * NPAD + rp * 32 + m;
+      soff[u] = g * LD + dtile * 16 + m;
+    }
+    double a0[2][NU], a1[2][NU], bx[2][NU], bp[2][NU];
+    auto load = [&](int set, int kc) __attribute__((always_inline)) {
+      const double* Tk = T + (size_t)(4 * kc) * NPAD;  // wave-uniform chunk base
+      const double* Sk = S + 4 * kc * LD;
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        a0[set][u] = Tk[toff[u]];
+        a1[set][u] = Tk[toff[u] + 16];
+        bx[set][u] = Sk[soff[u]];
+        bp[set][u] = Sk[soff[u] + DT * 16];
+      }
+    };
+    auto mma = [&](int set) __attribute__((always_inline)) {
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        acc[u][0] = mfma_f64(a0[set][u], bx[set][u], acc[u][0]);
+        acc[u][1] = mfma_f64(a0[set][u], bp[set][u], acc[u][1]);
+        acc[u][2] = mfma_f64(a1[set][u], bx[set][u], acc[u][2]);
+        acc[u][3] = mfma_f64(a1[set][u], bp[set][u], acc[u][3]);
+      }
+    };
+    load(0, 0);
+    for (int kc = 0; kc < nk; kc += 2) {
+      load(1, kc + 1);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(0);
+      __builtin_amdgcn_sched_barrier(0);
+      load(0, kc + 2 < nk ? kc + 2 : kc + 1);  // the last prefetch re-reads a valid chunk
+      __builtin_amdgcn_sched_barrier(0);
+      mma(1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
-  double a0[2][UPW], a1[2][UPW], bx[2][UPW], bp[2][UPW];
-  auto load = [&](int set, int kc) __attribute__((always_inline)) {
-    const double* Tk = T + (size_t)(4 * kc) * NPAD;  // wave-uniform chunk base
-    const double* Sk = S + 4 * kc * LD;
-#pragma unroll
-    for (int u = 0; u < UPW; ++u) {
-      a0[set][u] = Tk[toff[u]];
-      a1[set][u] = Tk[toff[u] + 16];
-      bx[set][u] = Sk[soff[u]];
-      bp[set][u] = Sk[soff[u] + DT * 16];
-    }
-  };
-  auto mma = [&](int set) __attribute__((always_inline)) {
-#pragma unroll
-    for (int u = 0; u < UPW; ++u) {
-      acc[u][0] = mfma_f64(a0[set][u], bx[set][u], acc[u][0]);
-      acc[u][1] = mfma_f64(a0[set][u], bp[set][u], acc[u][1]);
-      acc[u][2] = mfma_f64(a1[set][u], bx[set][u], acc[u][2]);
-      acc[u][3] = mfma_f64(a1[set][u], bp[set][u], acc[u][3]);
-    }
-  };
-  load(0, 0);
-  for (int kc = 0; kc < nk; kc += 2) {
-    load(1, kc + 1);
-    __builtin_amdgcn_sched_barrier(0);
-    mma(0);
-    __builtin_amdgcn_sched_barrier(0);
-    load(0, kc + 2 < nk ? kc + 2 : kc + 1);  // the last prefetch re-reads a valid chunk
-    __builtin_amdgcn_sched_barrier(0);
-    mma(1);
-    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int DT, int UPW, int W>
+__device__ __forceinline__ void gemm_T(const double* __restrict__ T, int NPAD, const double* S,
+                                       int LD, int wave, int m, int g, d4 (&acc)[UPW][4]) {
+  if constexpr (UPW == 1) {
+    // one slot: a wave past the last unit recomputes the last unit (its result is dropped
+    // by the consumers); the branch-free form keeps the one-unit kernels' registers
+    gemm_T_n<DT, 1, W, 1>(T, NPAD, S, LD, wave, m, g, acc);
+  } else {
+    const int units = (NPAD >> 5) * DT;
+    with_units<UPW>(active_units<UPW, W>(wave, units), [&](auto NU) __attribute__((always_inline)) {
+      gemm_T_n<DT, UPW, W, decltype(NU)::value>(T, NPAD, S, LD, wave, m, g, acc);
+    });
   }
 }
 
@@ -96,11 +128,10 @@ __device__ __forceinline__ void gemm_T(const double* __restrict__ T, int NPAD, c
 //   backward: P_l = sum_r Todd[r][l] y_{2r},  Q_l = sum_r Teven[r][l] y_{2r+1}   (S rows = modes)
 //             x_l = P_l + Q_l, x_{N-1-l} = P_l - Q_l                       (store_fold)
 // The loop is the ping-pong of gemm_T over HP/4 K-chunks: half of gemm_T's MFMAs.
-template <int DT, int UPW, int W, bool BWD>
-__device__ __forceinline__ void gemm_fold(const double* __restrict__ Tf, int N, const double* S,
-                                          int LD, int wave, int m, int g, d4 (&acc)[UPW][4]) {
+template <int DT, int UPW, int W, bool BWD, int NU>
+__device__ __forceinline__ void gemm_fold_n(const double* __restrict__ Tf, int N, const double* S,
+                                            int LD, int wave, int m, int g, d4 (&acc)[UPW][4]) {
   const int HP = ((N >> 1) + 15) / 16 * 16;
-  const int units = (HP >> 4) * DT;
   const size_t Q = (size_t)HP * HP;
   const double* To = Tf + (BWD ? 2 * Q : 0);
   const double* Te = To + Q;
@@ -108,56 +139,73 @@ __device__ __forceinline__ void gemm_fold(const double* __restrict__ Tf, int N, 
   for (int u = 0; u < UPW; ++u)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[u][c] = d4{0.0, 0.0, 0.0, 0.0};
-  const int nk = HP >> 2;  // a multiple of 4
-  int toff[UPW], soff[UPW];
+  if constexpr (NU > 0) {
+    const int nk = HP >> 2;  // a multiple of 4
+    const int units = (HP >> 4) * DT;
+    int toff[NU], soff[NU];
 #pragma unroll
-  for (int u = 0; u < UPW; ++u) {
-    const int unit = min(wave + u * W, units - 1);
-    const int rp = unit / DT, dtile = unit - rp * DT;
-    toff[u] = g * HP + rp * 16 + m;
-    soff[u] = dtile * 16 + m;
+    for (int u = 0; u < NU; ++u) {
+      const int unit = min(wave + u * W, units - 1);
+      const int rp = unit / DT, dtile = unit - rp * DT;
+      toff[u] = g * HP + rp * 16 + m;
+      soff[u] = dtile * 16 + m;
+    }
+    // T fragments (global, long latency) in ping-pong; S fragments (LDS) loaded just before
+    // their MFMAs, which keeps the operand registers at 8 per unit
+    double ao[2][NU], ae[2][NU];
+    auto loadA = [&](int set, int kc) __attribute__((always_inline)) {
+      const size_t tb = (size_t)(4 * kc) * HP;  // wave-uniform chunk base
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        ao[set][u] = To[tb + toff[u]];
+        ae[set][u] = Te[tb + toff[u]];
+      }
+    };
+    auto mma = [&](int set, int kc) __attribute__((always_inline)) {
+      const int k = 4 * kc + g;
+      const int ro = BWD ? 2 * k : k;                       // odd-half operand row
+      const int re = BWD ? 2 * k + 1 : max(N - 1 - k, 0);   // even-half operand row
+      double xo[NU], po[NU], xe[NU], pe[NU];
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        xo[u] = S[ro * LD + soff[u]];
+        po[u] = S[ro * LD + DT * 16 + soff[u]];
+        xe[u] = S[re * LD + soff[u]];
+        pe[u] = S[re * LD + DT * 16 + soff[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        acc[u][0] = mfma_f64(ao[set][u], xo[u], acc[u][0]);
+        acc[u][1] = mfma_f64(ao[set][u], po[u], acc[u][1]);
+        acc[u][2] = mfma_f64(ae[set][u], xe[u], acc[u][2]);
+        acc[u][3] = mfma_f64(ae[set][u], pe[u], acc[u][3]);
+      }
+    };
+    loadA(0, 0);
+    for (int kc = 0; kc < nk; kc += 2) {
+      loadA(1, kc + 1);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(0, kc);
+      __builtin_amdgcn_sched_barrier(0);
+      loadA(0, kc + 2 < nk ? kc + 2 : kc + 1);  // the last prefetch re-reads a valid chunk
+      __builtin_amdgcn_sched_barrier(0);
+      mma(1, kc + 1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
-  // T fragments (global, long latency) in ping-pong; S fragments (LDS) loaded just before
-  // their MFMAs, which keeps the operand registers at 8 per unit
-  double ao[2][UPW], ae[2][UPW];
-  auto loadA = [&](int set, int kc) __attribute__((always_inline)) {
-    const size_t tb = (size_t)(4 * kc) * HP;  // wave-uniform chunk base
-#pragma unroll
-    for (int u = 0; u < UPW; ++u) {
-      ao[set][u] = To[tb + toff[u]];
-      ae[set][u] = Te[tb + toff[u]];
-    }
-  };
-  auto mma = [&](int set, int kc) __attribute__((always_inline)) {
-    const int k = 4 * kc + g;
-    const int ro = BWD ? 2 * k : k;                       // odd-half operand row
-    const int re = BWD ? 2 * k + 1 : max(N - 1 - k, 0);   // even-half operand row
-    double xo[UPW], po[UPW], xe[UPW], pe[UPW];
-#pragma unroll
-    for (int u = 0; u < UPW; ++u) {
-      xo[u] = S[ro * LD + soff[u]];
-      po[u] = S[ro * LD + DT * 16 + soff[u]];
-      xe[u] = S[re * LD + soff[u]];
-      pe[u] = S[re * LD + DT * 16 + soff[u]];
-    }
-#pragma unroll
-    for (int u = 0; u < UPW; ++u) {
-      acc[u][0] = mfma_f64(ao[set][u], xo[u], acc[u][0]);
-      acc[u][1] = mfma_f64(ao[set][u], po[u], acc[u][1]);
-      acc[u][2] = mfma_f64(ae[set][u], xe[u], acc[u][2]);
-      acc[u][3] = mfma_f64(ae[set][u], pe[u], acc[u][3]);
-    }
-  };
-  loadA(0, 0);
-  for (int kc = 0; kc < nk; kc += 2) {
-    loadA(1, kc + 1);
-    __builtin_amdgcn_sched_barrier(0);
-    mma(0, kc);
-    __builtin_amdgcn_sched_barrier(0);
-    loadA(0, kc + 2 < nk ? kc + 2 : kc + 1);  // the last prefetch re-reads a valid chunk
-    __builtin_amdgcn_sched_barrier(0);
-    mma(1, kc + 1);
-    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int DT, int UPW, int W, bool BWD>
+__device__ __forceinline__ void gemm_fold(const double* __restrict__ Tf, int N, const double* S,
+                                          int LD, int wave, int m, int g, d4 (&acc)[UPW][4]) {
+  if constexpr (UPW == 1) {
+    gemm_fold_n<DT, 1, W, BWD, 1>(Tf, N, S, LD, wave, m, g, acc);  // as in gemm_T
+  } else {
+    const int HP = ((N >> 1) + 15) / 16 * 16;
+    const int units = (HP >> 4) * DT;
+    with_units<UPW>(active_units<UPW, W>(wave, units), [&](auto NU) __attribute__((always_inline)) {
+      gemm_fold_n<DT, UPW, W, BWD, decltype(NU)::value>(Tf, N, S, LD, wave, m, g, acc);
+    });
   }
 }
 
