@@ -121,10 +121,9 @@ __device__ __forceinline__ void gemm_T(const double* __restrict__ T, int NPAD, c
 // The folded transforms (even N, H = N/2, HP = H padded to 16).  A unit is a block of 16
 // rows r of both halves and one dof tile; acc[u][0..1] hold the x / p columns of the odd
 // half, acc[u][2..3] those of the even half.
-//   forward:  odd r  (mode 2r+1) = sum_k Todd[r][k]  (x_k + x_{N-1-k})      (S rows k)
-//             even r (mode 2r+2) = sum_k Teven[r][k] (x_k - x_{N-1-k})      (S rows N-1-k)
-//             after the pre-fold that leaves the sums in rows k < H, the differences in
-//             rows N-1-k;
+//   forward:  odd r  (mode 2r+1) = sum_k Todd[r][k]  (x_k + x_{N-1-k})
+//             even r (mode 2r+2) = sum_k Teven[r][k] (x_k - x_{N-1-k})
+//             the sum and difference are formed from the two S rows as they are loaded;
 //   backward: P_l = sum_r Todd[r][l] y_{2r},  Q_l = sum_r Teven[r][l] y_{2r+1}   (S rows = modes)
 //             x_l = P_l + Q_l, x_{N-1-l} = P_l - Q_l                       (store_fold)
 // The loop is the ping-pong of gemm_T over HP/4 K-chunks: half of gemm_T's MFMAs.
@@ -172,6 +171,16 @@ __device__ __forceinline__ void gemm_fold_n(const double* __restrict__ Tf, int N
         po[u] = S[ro * LD + DT * 16 + soff[u]];
         xe[u] = S[re * LD + soff[u]];
         pe[u] = S[re * LD + DT * 16 + soff[u]];
+        if constexpr (!BWD) {
+          // the fold in registers: rows k and N-1-k -> their sum (odd half) and difference
+          // (even half).  Rows k >= H meet zero columns of the tables.
+          const double sx = xo[u] + xe[u], dx = xo[u] - xe[u];
+          const double sp = po[u] + pe[u], dp = po[u] - pe[u];
+          xo[u] = sx;
+          xe[u] = dx;
+          po[u] = sp;
+          pe[u] = dp;
+        }
       }
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
@@ -206,20 +215,6 @@ __device__ __forceinline__ void gemm_fold(const double* __restrict__ Tf, int N, 
     with_units<UPW>(active_units<UPW, W>(wave, units), [&](auto NU) __attribute__((always_inline)) {
       gemm_fold_n<DT, UPW, W, BWD, decltype(NU)::value>(Tf, N, S, LD, wave, m, g, acc);
     });
-  }
-}
-
-// Pre-fold of the forward transform: rows k < H <- x_k + x_{N-1-k}, rows N-1-k <- the
-// difference, for every x and p column (each pair owned by one thread).
-template <int DT, int W>
-__device__ __forceinline__ void prefold(double* S, int N, int LD, int tid) {
-  constexpr int C2 = 2 * DT * 16;
-  const int H = N >> 1;
-  for (int e = tid; e < H * C2; e += W * 64) {
-    const int l = e / C2, c = e - l * C2;
-    const double a = S[l * LD + c], b = S[(N - 1 - l) * LD + c];
-    S[l * LD + c] = a + b;
-    S[(N - 1 - l) * LD + c] = a - b;
   }
 }
 
@@ -725,8 +720,6 @@ __global__ void __launch_bounds__(W * 64, (W == 4 && UPW == 1) ? 2 : 1)
       const bool fwd = p == PH_FWD_PILE || p == PH_FWD_A;
       if constexpr (FOLD) {
         if (fwd) {
-          prefold<DT, W>(S, N, LD, tid);
-          __syncthreads();
           gemm_fold<DT, UPW, W, false>(A.Tf, N, S, LD, wave, m, g, acc);
         } else {
           gemm_fold<DT, UPW, W, true>(A.Tf, N, S, LD, wave, m, g, acc);
