@@ -141,9 +141,28 @@ __global__ void __launch_bounds__(kLbThreads) lb_init_kernel(LbArgs A) {
   }
 }
 
+// f(q) for q = Q0 .. P-1 (compile-time), in chunks of 8 (4 above P = 16) with a scheduling
+// barrier after each:
+// the loads of one chunk are in flight together and the registers they take stay bounded
+template <int Q0, int P, typename F>
+__device__ __forceinline__ void each(F&& f) {
+  constexpr int C = P > 16 ? 4 : 8;
+  if constexpr (Q0 < P) {
+#pragma unroll
+    for (int q = Q0; q < Q0 + C && q < P; ++q) f(q);
+    __builtin_amdgcn_sched_barrier(0);
+    each<Q0 + C, P>(f);
+  }
+}
+
 // d = -H g by the two-loop recursion, the descent check, and the first trial point.  Each
 // pass over the chain fuses one correction's update with the next correction's dot product,
 // so the recursion reads the direction 2k + 1 times for k pairs.
+// P > 0: the direction stays in registers (at most P elements per thread, element
+// j = threadIdx.x + q kLbThreads in slot q) through the 2k passes of the recursion, which then
+// read only the pairs, with every load of a pass in flight at once; the arithmetic and its
+// order are those of the streaming form (P = 0), which keeps D in HBM for longer chains.
+template <int P>
 __global__ void __launch_bounds__(kLbThreads) lb_direction_kernel(LbArgs A) {
   __shared__ double red[kLbWaves];
   __shared__ double alpha[kMaxM + 1];
@@ -172,6 +191,101 @@ __global__ void __launch_bounds__(kLbThreads) lb_direction_kernel(LbArgs A) {
       D[j] = v;
       part += G[j] * v;
       XT[j] = X[j] + a0 * v;
+    }
+  } else if constexpr (P > 0) {
+    // loads at a clamped index and selects, no branch per element: the loads of a chunk
+    // issue back to back.  Slots past the chain end keep d = 0 and add nothing.  The thread
+    // index is re-read from an opaque copy in every pass, so the element offsets are
+    // recomputed where they are used instead of being held (and spilled) across passes.
+    int t = threadIdx.x;
+    auto fresh = [&]() __attribute__((always_inline)) {
+      t = threadIdx.x;
+      asm volatile("" : "+v"(t));
+    };
+    auto at = [&](int q) __attribute__((always_inline)) { return min(t + q * kLbThreads, L - 1); };
+    auto in = [&](int q) __attribute__((always_inline)) { return t + q * kLbThreads < L; };
+    double d[P];
+    {
+      const double* Sk = Sv(k - 1);
+      fresh();
+      each<0, P>([&](int q) __attribute__((always_inline)) {
+        const int j = at(q);
+        const double v = Sk[j] * G[j] + part;
+        d[q] = 0.0;
+        part = in(q) ? v : part;
+      });
+    }
+    for (int i = k - 1; i >= 0; --i) {
+      const double al = rho[slot(i)] * lb_sum(part, red);
+      if (threadIdx.x == 0) alpha[i] = al;
+      const double* Yp = Yv(i);
+      const bool first = i == k - 1;  // the first backward update reads the gradient
+      part = 0.0;
+      if (i > 0) {
+        const double* Sn = Sv(i - 1);
+        if (first) {
+          fresh();
+          each<0, P>([&](int q) __attribute__((always_inline)) {
+            const int j = at(q);
+            const double v = G[j] - al * Yp[j];
+            const double w = Sn[j] * v + part;
+            d[q] = in(q) ? v : d[q];
+            part = in(q) ? w : part;
+          });
+        } else {
+          fresh();
+          each<0, P>([&](int q) __attribute__((always_inline)) {
+            const int j = at(q);
+            const double v = d[q] - al * Yp[j];
+            const double w = Sn[j] * v + part;
+            d[q] = in(q) ? v : d[q];
+            part = in(q) ? w : part;
+          });
+        }
+      } else {
+        // last backward update, the initial scaling, and the first forward dot product
+        fresh();
+        each<0, P>([&](int q) __attribute__((always_inline)) {
+          const int j = at(q);
+          const double y = Yp[j];
+          const double src = first ? G[j] : d[q];
+          const double v = (src - al * y) * gam;
+          const double w = y * v + part;
+          d[q] = in(q) ? v : d[q];
+          part = in(q) ? w : part;
+        });
+      }
+    }
+    __syncthreads();  // alpha[] complete before the forward loop reads it
+    for (int i = 0; i < k; ++i) {
+      const double coef = alpha[i] - rho[slot(i)] * lb_sum(part, red);
+      const double* Sp = Sv(i);
+      part = 0.0;
+      if (i < k - 1) {
+        const double* Yn = Yv(i + 1);
+        fresh();
+        each<0, P>([&](int q) __attribute__((always_inline)) {
+          const int j = at(q);
+          const double v = d[q] + coef * Sp[j];
+          const double w = Yn[j] * v + part;
+          d[q] = in(q) ? v : d[q];
+          part = in(q) ? w : part;
+        });
+      } else {
+        // last forward update, the sign, the slope along d and the unit trial step
+        fresh();
+        each<0, P>([&](int q) __attribute__((always_inline)) {
+          const int j = at(q);
+          const double v = -(d[q] + coef * Sp[j]);
+          const double w = G[j] * v + part;
+          const double xt = X[j] + v;
+          part = in(q) ? w : part;
+          if (in(q)) {
+            D[j] = v;
+            XT[j] = xt;
+          }
+        });
+      }
     }
   } else {
     {
@@ -437,7 +551,12 @@ extern "C" int rp_instanton(rp_ring* r, double pgtol, double factr, int m, int m
   if ((rc = ring_upload_x(r, x_inout)) || (rc = ring_eval(r, true))) return rc;
   lb_init_kernel<<<nc, kLbThreads, 0, rb.s>>>(A);
   RP_HIP(hipGetLastError());
-  lb_direction_kernel<<<nc, kLbThreads, 0, rb.s>>>(A);
+  // direction kernel by the longest chain's elements per thread
+  const int nmax = *std::max_element(ring_n(r).begin(), ring_n(r).end());
+  const int per = (nmax * rb.D + kLbThreads - 1) / kLbThreads;
+  auto direction = per <= 8 ? lb_direction_kernel<8> : per <= 16 ? lb_direction_kernel<16>
+                 : per <= 28 ? lb_direction_kernel<28> : lb_direction_kernel<0>;
+  direction<<<nc, kLbThreads, 0, rb.s>>>(A);
   RP_HIP(hipGetLastError());
   std::vector<int> hs(nc);
   const long cap = maxiter > 0 ? (long)maxiter * kMaxLS + 1 : kMaxRounds;
@@ -454,7 +573,7 @@ extern "C" int rp_instanton(rp_ring* r, double pgtol, double factr, int m, int m
     if ((rc = ring_eval(r, true))) return rc;
     lb_feed_kernel<<<nc, kLbThreads, 0, rb.s>>>(A);
     RP_HIP(hipGetLastError());
-    lb_direction_kernel<<<nc, kLbThreads, 0, rb.s>>>(A);
+    direction<<<nc, kLbThreads, 0, rb.s>>>(A);
     RP_HIP(hipGetLastError());
   }
   if (capped) {
