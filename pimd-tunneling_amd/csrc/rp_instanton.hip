@@ -380,7 +380,12 @@ __global__ void __launch_bounds__(kLbThreads) lb_feed_kernel(LbArgs A) {
   const int64_t base = A.off[c];
   const int L = A.n[c] * A.D;
   double part = 0.0;
-  for (int j = threadIdx.x; j < L; j += kLbThreads) part += A.GT[base + j] * A.Dv[base + j];
+  {
+    const double* __restrict__ gt = A.GT + base;
+    const double* __restrict__ dv = A.Dv + base;
+#pragma unroll 8
+    for (int j = threadIdx.x; j < L; j += kLbThreads) part += gt[j] * dv[j];
+  }
   const double dgt = lb_sum(part, red);
   const double ft = A.E[c];
   // every lane runs the same scalar decision
@@ -438,13 +443,15 @@ __global__ void __launch_bounds__(kLbThreads) lb_feed_kernel(LbArgs A) {
       }
     }
   }
-  double* X = A.X + base;
-  double* G = A.G + base;
-  double* Glo = A.Glo + base;
-  double* Gp = A.Gprev + base;
-  double* XT = A.XT + base;
-  const double* GT = A.GT + base;
-  const double* Dv = A.Dv + base;
+  // distinct vectors: the loads of several iterations may run ahead of earlier stores
+  double* __restrict__ X = A.X + base;
+  double* __restrict__ G = A.G + base;
+  double* __restrict__ Glo = A.Glo + base;
+  double* __restrict__ Gp = A.Gprev + base;
+  double* __restrict__ XT = A.XT + base;
+  const double* __restrict__ GT = A.GT + base;
+  const double* __restrict__ Dv = A.Dv + base;
+#pragma unroll 4
   for (int j = threadIdx.x; j < L; j += kLbThreads) {
     if (cp & CP_PREV_TO_LO) Glo[j] = Gp[j];
     if (cp & CP_G_TO_LO) Glo[j] = G[j];
@@ -452,16 +459,18 @@ __global__ void __launch_bounds__(kLbThreads) lb_feed_kernel(LbArgs A) {
     if (cp & CP_GT_TO_PREV) Gp[j] = GT[j];
   }
   if (act == ACT_TRIAL) {
+#pragma unroll 8
     for (int j = threadIdx.x; j < L; j += kLbThreads) XT[j] = X[j] + a_new * Dv[j];
     s.a_cur = a_new;
   } else if (act == ACT_ACCEPT_CUR || act == ACT_ACCEPT_LO) {
     const int m1 = A.m + 1;
     const int slot = (s.head + s.count) % m1;
-    double* Sp = A.S + (int64_t)slot * A.tot + base;
-    double* Yp = A.Y + (int64_t)slot * A.tot + base;
+    double* __restrict__ Sp = A.S + (int64_t)slot * A.tot + base;
+    double* __restrict__ Yp = A.Y + (int64_t)slot * A.tot + base;
     const bool cur = act == ACT_ACCEPT_CUR;
     const double alo = s.a_lo;
     double psy = 0.0, pyy = 0.0, pg = 0.0;
+#pragma unroll 4
     for (int j = threadIdx.x; j < L; j += kLbThreads) {
       const double xa = cur ? XT[j] : X[j] + alo * Dv[j];
       const double ga = cur ? GT[j] : Glo[j];

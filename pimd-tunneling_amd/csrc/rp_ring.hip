@@ -42,21 +42,10 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 
 constexpr int kRingTile = 128;  // beads per workgroup of the per-bead ring kernels
 
-// The potential as seen by one workgroup: the EVB matrices (12.5 KB at 27 dof) are staged
-// in LDS once per workgroup, every other kind keeps its few parameters in global memory.
-template <int KIND, int MAXD>
-struct StagedPes {
-  static constexpr int kLds = KIND == PES_EVB ? evb_dev_size<MAXD>() : 1;
-  __device__ static PesDev stage(const PesDev& P, double* lds) {
-    PesDev Q = P;
-    if constexpr (KIND == PES_EVB) {
-      for (int e = threadIdx.x; e < kLds; e += blockDim.x) lds[e] = P.dp[e];
-      Q.dp = lds;
-    }
-    __syncthreads();
-    return Q;
-  }
-};
+// The potential's parameters are read from global memory at wave-uniform addresses (the
+// EVB row i of a well, every dof j), which the compiler turns into scalar loads: the FMAs
+// take them from SGPRs.  A copy staged in LDS made every lane read every element, and the
+// LDS return bandwidth, not the FP64 pipes, then bounded the EVB evaluation.
 
 // U_M and its adjoint gradient (UMforceenergy, instantonmod.f90:93-151):
 // U = sum_i V(x_i) + sum_{i<n} (m/2bn^2)|x_{i+1}-x_i|^2 [+ (m/2bn^2)(|x_1-a|^2 + |b-x_n|^2)]
@@ -67,12 +56,11 @@ __global__ void __launch_bounds__(kRingTile) ring_um_kernel(RingArgs A, const do
                                                             double* __restrict__ grad,
                                                             double* __restrict__ energy) {
   __shared__ double red[kRingTile];
-  __shared__ double pl[StagedPes<KIND, MAXD>::kLds];
   const int c = blockIdx.y;
   const int n = A.n[c], D = A.D;
   const int i = blockIdx.x * kRingTile + threadIdx.x;
   if (blockIdx.x * kRingTile >= n) return;  // the same for the whole workgroup
-  const PesDev P = StagedPes<KIND, MAXD>::stage(A.pes, pl);
+  const PesDev P = A.pes;
   const double bn2 = A.betan[c] * A.betan[c];
   const double* xc = x + A.off[c];
   const double* a = A.ab + (size_t)c * 2 * D;
@@ -121,12 +109,11 @@ template <int KIND, int MAXD>
 __global__ void __launch_bounds__(kRingTile) ring_hvp_kernel(RingArgs A, const double* __restrict__ x,
                                                              const double* __restrict__ v,
                                                              double* __restrict__ hv) {
-  __shared__ double pl[StagedPes<KIND, MAXD>::kLds];
   const int c = blockIdx.y;
   const int n = A.n[c], D = A.D;
   const int i = blockIdx.x * kRingTile + threadIdx.x;
   if (blockIdx.x * kRingTile >= n) return;  // the same for the whole workgroup
-  const PesDev P = StagedPes<KIND, MAXD>::stage(A.pes, pl);
+  const PesDev P = A.pes;
   const double bn2 = A.betan[c] * A.betan[c];
   const int64_t o = A.off[c];
   if (i >= n) return;
@@ -157,11 +144,10 @@ template <int KIND, int MAXD>
 __global__ void __launch_bounds__(128) ring_hblock_kernel(RingArgs A, const double* __restrict__ x,
                                                           int single_first,
                                                           double* __restrict__ blocks) {
-  __shared__ double pl[StagedPes<KIND, MAXD>::kLds];
   const int c = blockIdx.y;
   const int n = A.n[c], D = A.D;
   if ((int)(blockIdx.x * blockDim.x) >= n * D) return;  // the same for the whole workgroup
-  const PesDev P = StagedPes<KIND, MAXD>::stage(A.pes, pl);
+  const PesDev P = A.pes;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int bead = t / D, i = t - bead * D;
   if (bead >= n) return;
