@@ -131,6 +131,34 @@ def test_graph_replays_match_direct_launches(thermostat):
         assert np.array_equal(bs[0].kick_cursors(), bs[1].kick_cursors())
 
 
+@pytest.mark.parametrize("thermostat", [2, 1])
+def test_wave_counts_agree(thermostat):
+    """Both workgroup shapes give bit for bit the same states, TI sums and stream positions:
+    4 waves and 8 (families of at most 16 MFMA units, here all): idle unit slots at N = 24,
+    32, 33 and 64, and two units per wave at N = 128 with 4 waves."""
+    sp = specs.evb()
+    pes = Pes(sp)
+    pols = _polymers(sp, [32, 24, 64, 128, 33], xis=[0.2, 0.5, 0.7, 0.9, 0.4])
+    res = []
+    for waves in (4, 8):
+        b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, thermostat=thermostat, noutput=5, seed=13)
+        b.set_launch(steps_per_launch=8, waves=waves)
+        for k, pol in enumerate(pols):
+            x, p = _state(sp, pol["n"], 60 + k)
+            b.set_state(k, x, p)
+        b.run(20, imin=3)
+        sums, cnt, _ = b.ti_readout()
+        res.append(([b.get_state(k) for k in range(len(pols))], sums, cnt, b.cursors(),
+                    b.kick_cursors()))
+    for r in res[1:]:
+        for k in range(len(pols)):
+            assert np.array_equal(r[0][k][0], res[0][0][k][0]), k
+            assert np.array_equal(r[0][k][1], res[0][0][k][1]), k
+        for i in (1, 2, 3, 4):
+            assert np.array_equal(r[i], res[0][i])
+    assert np.all(res[0][2] == 17)
+
+
 def test_batch_init_path_and_reinit():
     sp = specs.evb()
     pes = Pes(sp)
