@@ -104,6 +104,159 @@ __global__ void __launch_bounds__(kRingTile) ring_um_kernel(RingArgs A, const do
   if (threadIdx.x == 0) atomicAdd(&energy[c], red[0]);
 }
 
+// U_M and its adjoint gradient for the EVB on the matrix cores.  A wave takes a tile of 16
+// beads: both wells' coordinates y = Wt x - cw are one GEMM (rows: 2 x 16 RT well rows; K:
+// the dofs) and the mixed gradient wa WtA^T hA + wb WtB^T hB a second GEMM whose B operand
+// is the first one's accumulators, scaled in place (item j of row tile rt holds row
+// 16 rt + g + 4 j of bead m, the operand of K-chunk 4 rt + j).  The spring terms are then
+// added per (bead, dof) as in ring_um_kernel.  The sums run in another order than that
+// kernel's per-bead loop, so the two agree to rounding, not bit for bit.
+typedef double rd4 __attribute__((ext_vector_type(4)));
+constexpr int kUmWaves = 4;
+
+template <int M>
+__global__ void __launch_bounds__(kUmWaves * 64) ring_um_evb_kernel(RingArgs A,
+                                                                    const double* __restrict__ x,
+                                                                    double* __restrict__ grad,
+                                                                    double* __restrict__ energy) {
+  constexpr int RT = (M + 15) / 16;  // row tiles per well, and dof tiles
+  constexpr int DC = 16 * RT;        // padded dofs
+  constexpr int LX = DC + 1;         // LDS stride of one bead
+  constexpr int KY = 4 * RT;         // K-chunks of the y GEMM
+  constexpr int KG = 8 * RT;         // K-chunks of the gradient GEMM
+  constexpr int NT = kUmWaves * 64;
+  constexpr int TB = kUmWaves * 16;  // beads per workgroup
+  __shared__ double pl[evb_dev_size<M>()];
+  __shared__ double xs[TB * LX];  // x of the workgroup's beads, then their potential gradient
+  __shared__ double vpot[TB];
+  __shared__ double red[NT];
+  const int c = blockIdx.y;
+  const int n = A.n[c], D = A.D;
+  const int b0 = blockIdx.x * TB;
+  if (b0 >= n) return;  // the same for the whole workgroup
+  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const double* xc = x + A.off[c];
+  for (int e = tid; e < evb_dev_size<M>(); e += NT) pl[e] = A.pes.dp[e];
+  for (int e = tid; e < TB * DC; e += NT) {
+    const int d = e / TB, bl = e - d * TB;  // consecutive threads read consecutive beads
+    const int i = b0 + bl;
+    xs[bl * LX + d] = (d < D && i < n) ? xc[(size_t)d * n + i] : 0.0;
+  }
+  __syncthreads();
+  {
+    const double kappa = pl[0];
+    const double* WA = pl + 1;
+    const double* cA = WA + M * M;
+    const double* WB = cA + M;
+    const double* cB = WB + M * M;
+    const double* lam = cB + M;
+    const double* cub = lam + M;
+    const int tb = wave * 16;
+    rd4 ay[2 * RT];
+#pragma unroll
+    for (int r = 0; r < 2 * RT; ++r) ay[r] = rd4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kc = 0; kc < KY; ++kc) {
+      const int j = 4 * kc + g;
+      const double bx = xs[(tb + m) * LX + j];
+#pragma unroll
+      for (int r = 0; r < 2 * RT; ++r) {
+        const int i = (r % RT) * 16 + m;
+        const double* Wt = r < RT ? WA : WB;
+        const double w = Wt[min(i, M - 1) * M + min(j, M - 1)];
+        ay[r] = __builtin_amdgcn_mfma_f64_16x16x4f64((i < M && j < M) ? w : 0.0, bx, ay[r], 0, 0, 0);
+      }
+    }
+    double va = 0.0, vb = 0.0;
+#pragma unroll
+    for (int r = 0; r < 2 * RT; ++r)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const int i = (r % RT) * 16 + g + 4 * jj;
+        const bool ok = i < M;
+        const int ic = min(i, M - 1);
+        const double cw = ok ? (r < RT ? cA[ic] : cB[ic]) : 0.0;
+        const double li = ok ? lam[ic] : 0.0, ci = ok ? cub[ic] : 0.0;
+        const double y = ay[r][jj] - cw, y2 = y * y;
+        const double v = (0.5 * li) * y2 + ci * y2 * y;
+        if (r < RT)
+          va += v;
+        else
+          vb += v;
+        ay[r][jj] = li * y + (3.0 * ci) * y2;
+      }
+    va += __shfl_xor(va, 16);
+    va += __shfl_xor(va, 32);
+    vb += __shfl_xor(vb, 16);
+    vb += __shfl_xor(vb, 32);
+    const double dl = va - vb, R = sqrt(0.25 * dl * dl + kappa * kappa), f = dl / (4.0 * R);
+    const double wa = 0.5 - f, wb = 0.5 + f;
+#pragma unroll
+    for (int r = 0; r < 2 * RT; ++r) ay[r] *= (r < RT ? wa : wb);
+    rd4 ag[RT];
+#pragma unroll
+    for (int d = 0; d < RT; ++d) ag[d] = rd4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kc = 0; kc < KG; ++kc) {
+      const int r = kc / 4, jj = kc % 4;
+      const int i = (r % RT) * 16 + 4 * jj + g;
+      const double* Wt = r < RT ? WA : WB;
+      const double bh = ay[r][jj];
+#pragma unroll
+      for (int d = 0; d < RT; ++d) {
+        const int dof = d * 16 + m;
+        const double w = Wt[min(i, M - 1) * M + min(dof, M - 1)];
+        ag[d] = __builtin_amdgcn_mfma_f64_16x16x4f64((i < M && dof < M) ? w : 0.0, bh, ag[d], 0, 0, 0);
+      }
+    }
+    // this wave's own rows of xs: its reads above come first in LDS order
+    if (g == 0) vpot[tb + m] = (0.5 * (va + vb) - R) - A.pes.v0;  // as eval_kind
+#pragma unroll
+    for (int d = 0; d < RT; ++d)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) xs[(tb + m) * LX + d * 16 + g + 4 * jj] = ag[d][jj];
+  }
+  __syncthreads();
+  // springs per (bead, dof), as in ring_um_kernel
+  const double bn2 = A.betan[c] * A.betan[c];
+  const double* a = A.ab + (size_t)c * 2 * D;
+  const double* b = a + D;
+  double e = 0.0;
+  for (int t = tid; t < TB * D; t += NT) {
+    const int d = t / TB, bl = t - d * TB;
+    const int i = b0 + bl;
+    if (i >= n) continue;
+    const double m_ = A.massd[d];
+    const double* xd = xc + (size_t)d * n;
+    const double xi = xd[i];
+    if (d == 0) e += vpot[bl];
+    if (i < n - 1) {
+      const double dx = xd[i + 1] - xi;
+      e += (0.5 * m_ / bn2) * dx * dx;
+    }
+    double sg;
+    if (n == 1)
+      sg = A.fixedends ? m_ * (2.0 * xi - a[d] - b[d]) / bn2 : 0.0;
+    else if (i == 0)
+      sg = A.fixedends ? m_ * (2.0 * xi - a[d] - xd[1]) / bn2 : m_ * (xi - xd[1]) / bn2;
+    else if (i == n - 1)
+      sg = A.fixedends ? m_ * (2.0 * xi - xd[n - 2] - b[d]) / bn2 : m_ * (xi - xd[n - 2]) / bn2;
+    else
+      sg = m_ * (2.0 * xi - xd[i - 1] - xd[i + 1]) / bn2;
+    if (A.fixedends && i == 0) e += (0.5 * m_ / bn2) * (xi - a[d]) * (xi - a[d]);
+    if (A.fixedends && i == n - 1) e += (0.5 * m_ / bn2) * (b[d] - xi) * (b[d] - xi);
+    if (grad) grad[A.off[c] + (size_t)d * n + i] = sg + xs[bl * LX + d];
+  }
+  red[tid] = e;
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if (tid < st) red[tid] += red[tid + st];
+    __syncthreads();
+  }
+  if (tid == 0) atomicAdd(&energy[c], red[0]);
+}
+
 // Hessian-vector product of U_M: springs exactly, potential by forward-over-reverse.
 template <int KIND, int MAXD>
 __global__ void __launch_bounds__(kRingTile) ring_hvp_kernel(RingArgs A, const double* __restrict__ x,
@@ -484,8 +637,15 @@ int ring_eval(rp_ring* r, bool want_grad) {
   RP_HIP(hipMemsetAsync(r->d_e.p, 0, sizeof(double) * r->nchain, r->s));
   const dim3 grid((unsigned)((r->nmax + kRingTile - 1) / kRingTile), (unsigned)r->nchain);
   return dispatch_pes(r->pes, [&]<int K, int M>() -> int {
-    ring_um_kernel<K, M><<<grid, kRingTile, 0, r->s>>>(a, r->d_x.p,
+    if constexpr (K == PES_EVB) {  // the EVB's two GEMMs per bead tile on the matrix cores
+      constexpr int TB = kUmWaves * 16;
+      const dim3 g2((unsigned)((r->nmax + TB - 1) / TB), (unsigned)r->nchain);
+      ring_um_evb_kernel<M><<<g2, kUmWaves * 64, 0, r->s>>>(a, r->d_x.p,
                                                               want_grad ? r->d_g.p : nullptr, r->d_e.p);
+    } else {
+      ring_um_kernel<K, M><<<grid, kRingTile, 0, r->s>>>(a, r->d_x.p,
+                                                                want_grad ? r->d_g.p : nullptr, r->d_e.p);
+    }
     RP_HIP(hipGetLastError());
     return RP_OK;
   });
