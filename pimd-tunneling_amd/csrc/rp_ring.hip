@@ -127,7 +127,8 @@ __global__ void __launch_bounds__(kUmWaves * 64) ring_um_evb_kernel(RingArgs A,
   constexpr int NT = kUmWaves * 64;
   constexpr int TB = kUmWaves * 16;  // beads per workgroup
   __shared__ double pl[evb_dev_size<M>()];
-  __shared__ double xs[TB * LX];  // x of the workgroup's beads, then their potential gradient
+  __shared__ double xs[(TB + 2) * LX];  // x of beads b0-1 .. b0+TB (row 0 is bead b0-1)
+  __shared__ double gp[TB * LX];        // the potential gradient of beads b0 .. b0+TB-1
   __shared__ double vpot[TB];
   __shared__ double red[NT];
   const int c = blockIdx.y;
@@ -136,12 +137,18 @@ __global__ void __launch_bounds__(kUmWaves * 64) ring_um_evb_kernel(RingArgs A,
   if (b0 >= n) return;  // the same for the whole workgroup
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const double* xc = x + A.off[c];
-  for (int e = tid; e < evb_dev_size<M>(); e += NT) pl[e] = A.pes.dp[e];
-  for (int e = tid; e < TB * DC; e += NT) {
-    const int d = e / TB, bl = e - d * TB;  // consecutive threads read consecutive beads
-    const int i = b0 + bl;
-    xs[bl * LX + d] = (d < D && i < n) ? xc[(size_t)d * n + i] : 0.0;
+  const double* __restrict__ xc = x + A.off[c];
+  {
+    // distinct global sources and LDS targets: the loads of several iterations are in flight
+    const double* __restrict__ dp = A.pes.dp;
+#pragma unroll 4
+    for (int e = tid; e < evb_dev_size<M>(); e += NT) pl[e] = dp[e];
+#pragma unroll 4
+    for (int e = tid; e < (TB + 2) * DC; e += NT) {
+      const int d = e / (TB + 2), bl = e - d * (TB + 2);  // consecutive threads, consecutive beads
+      const int i = b0 - 1 + bl;
+      xs[bl * LX + d] = (d < D && i >= 0 && i < n) ? xc[(size_t)d * n + i] : 0.0;
+    }
   }
   __syncthreads();
   {
@@ -159,7 +166,7 @@ __global__ void __launch_bounds__(kUmWaves * 64) ring_um_evb_kernel(RingArgs A,
 #pragma unroll
     for (int kc = 0; kc < KY; ++kc) {
       const int j = 4 * kc + g;
-      const double bx = xs[(tb + m) * LX + j];
+      const double bx = xs[(tb + m + 1) * LX + j];
 #pragma unroll
       for (int r = 0; r < 2 * RT; ++r) {
         const int i = (r % RT) * 16 + m;
@@ -210,15 +217,14 @@ __global__ void __launch_bounds__(kUmWaves * 64) ring_um_evb_kernel(RingArgs A,
         ag[d] = __builtin_amdgcn_mfma_f64_16x16x4f64((i < M && dof < M) ? w : 0.0, bh, ag[d], 0, 0, 0);
       }
     }
-    // this wave's own rows of xs: its reads above come first in LDS order
     if (g == 0) vpot[tb + m] = (0.5 * (va + vb) - R) - A.pes.v0;  // as eval_kind
 #pragma unroll
     for (int d = 0; d < RT; ++d)
 #pragma unroll
-      for (int jj = 0; jj < 4; ++jj) xs[(tb + m) * LX + d * 16 + g + 4 * jj] = ag[d][jj];
+      for (int jj = 0; jj < 4; ++jj) gp[(tb + m) * LX + d * 16 + g + 4 * jj] = ag[d][jj];
   }
   __syncthreads();
-  // springs per (bead, dof), as in ring_um_kernel
+  // springs per (bead, dof), as in ring_um_kernel; the neighbours come from the LDS rows
   const double bn2 = A.betan[c] * A.betan[c];
   const double* a = A.ab + (size_t)c * 2 * D;
   const double* b = a + D;
@@ -228,25 +234,26 @@ __global__ void __launch_bounds__(kUmWaves * 64) ring_um_evb_kernel(RingArgs A,
     const int i = b0 + bl;
     if (i >= n) continue;
     const double m_ = A.massd[d];
-    const double* xd = xc + (size_t)d * n;
-    const double xi = xd[i];
+    const double xi = xs[(bl + 1) * LX + d];
+    const double xl = xs[bl * LX + d];        // bead i - 1 (0 before the first bead)
+    const double xr = xs[(bl + 2) * LX + d];  // bead i + 1 (0 after the last bead)
     if (d == 0) e += vpot[bl];
     if (i < n - 1) {
-      const double dx = xd[i + 1] - xi;
+      const double dx = xr - xi;
       e += (0.5 * m_ / bn2) * dx * dx;
     }
     double sg;
     if (n == 1)
       sg = A.fixedends ? m_ * (2.0 * xi - a[d] - b[d]) / bn2 : 0.0;
     else if (i == 0)
-      sg = A.fixedends ? m_ * (2.0 * xi - a[d] - xd[1]) / bn2 : m_ * (xi - xd[1]) / bn2;
+      sg = A.fixedends ? m_ * (2.0 * xi - a[d] - xr) / bn2 : m_ * (xi - xr) / bn2;
     else if (i == n - 1)
-      sg = A.fixedends ? m_ * (2.0 * xi - xd[n - 2] - b[d]) / bn2 : m_ * (xi - xd[n - 2]) / bn2;
+      sg = A.fixedends ? m_ * (2.0 * xi - xl - b[d]) / bn2 : m_ * (xi - xl) / bn2;
     else
-      sg = m_ * (2.0 * xi - xd[i - 1] - xd[i + 1]) / bn2;
+      sg = m_ * (2.0 * xi - xl - xr) / bn2;
     if (A.fixedends && i == 0) e += (0.5 * m_ / bn2) * (xi - a[d]) * (xi - a[d]);
     if (A.fixedends && i == n - 1) e += (0.5 * m_ / bn2) * (b[d] - xi) * (b[d] - xi);
-    if (grad) grad[A.off[c] + (size_t)d * n + i] = sg + xs[bl * LX + d];
+    if (grad) grad[A.off[c] + (size_t)d * n + i] = sg + gp[bl * LX + d];
   }
   red[tid] = e;
   __syncthreads();
