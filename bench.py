@@ -282,8 +282,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): every trajectory "
-              "is independent, so several GPUs would each run this same batch; see DESIGN.md "
-              "for how the path shards.", file=sys.stderr)
+              "is independent, so the tasks would be dealt out to the GPUs (each its own batch "
+              "of whole polymers) with only the TI sums gathered at the end.", file=sys.stderr)
         sys.exit(2)
 
     import rptunnel
