@@ -145,10 +145,10 @@ static int batch_flags(rp_batch* b) {
 
 // Waves per workgroup by default: 8 (two per SIMD, which hides the latency of the
 // elementwise phases) from 8 to 16 MFMA units, 4 otherwise.  Measured on the 27-dof EVB
-// (ms per 8-step launch of 5632 polymers, 4 / 8 waves): 6 units 3.52 / 3.86 (two 4-wave
-// workgroups of two units per wave share a CU), 8 units 5.50 / 4.45, 12 units 9.09 / 7.63,
-// 16 units 13.5 / 11.3.  Four 2-wave workgroups per CU for 2 units (no idle wave) measured
-// 1.45 ms against 1.36 for two 4-wave ones.
+// (ms per 8-step launch of 5632 polymers, 4 / 8 waves): 2 units 1.37 / 2.15, 4 units
+// 1.69 / 2.68, 6 units 3.43 / 3.85 (two 4-wave workgroups share a CU), 8 units 5.30 / 4.37,
+// 12 units 8.88 / 7.51, 16 units 13.4 / 11.3.  Four 2-wave workgroups per CU for 2 units (no
+// idle wave) measured 1.45 ms against 1.36 for two 4-wave ones.
 static int default_waves(int DT, int NPAD) {
   const int units = (NPAD >> 5) * DT;
   return (units >= 8 && units <= 16) ? 8 : 4;
