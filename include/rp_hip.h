@@ -167,6 +167,10 @@ int rp_batch_get_kick_cursor(rp_batch *b, int64_t *kick_cursor);
 int rp_batch_set_cursor(rp_batch *b, const int64_t *cursor, const int64_t *kick_cursor);
 /* families of equal (n, betan), one fused-kernel launch each: n[f], count[f] */
 int rp_batch_families(rp_batch *b, int *nfam, int *n, int *count);
+/* the fused-kernel instance family `fam` runs: waves per workgroup, MFMA units per wave
+ * (a unit is 32 normal modes x 16 dofs of x and p) and whether its transforms are folded
+ * into two half-size products (even n) */
+int rp_batch_family_shape(rp_batch *b, int fam, int *waves, int *units_per_wave, int *folded);
 /* launch shape: steps per launch (>= 1; 0 keeps the current value) and waves per
  * polymer workgroup (4 or 8 for every family; 0 keeps each family's value; 8 applies to
  * families of at most 16 MFMA units, the others keep 4).  rp_batch_autotune times, on the

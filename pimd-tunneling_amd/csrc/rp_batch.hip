@@ -820,6 +820,19 @@ extern "C" int rp_batch_last_kernel_ms(rp_batch* b, double* avg_ms, int64_t* lau
   return RP_OK;
 }
 
+extern "C" int rp_batch_family_shape(rp_batch* b, int fam, int* waves, int* units_per_wave,
+                                     int* folded) {
+  RP_REQUIRE(b && fam >= 0 && fam < (int)b->fam.size(), "rp_batch_family_shape: bad arguments");
+  const Family& f = b->fam[fam];
+  // the kernel instance launch_batch_thermo picks: units of 32 modes x 16 dofs (folded: 16
+  // rows of each half x 16 dofs; the same count), dealt over the waves of a workgroup
+  const int units = (f.NPAD >> 5) * b->DT;
+  if (waves) *waves = f.waves;
+  if (units_per_wave) *units_per_wave = (units + f.waves - 1) / f.waves;
+  if (folded) *folded = (f.N % 2 == 0) && f.d_Tf.p != nullptr && b->pes->kind != PES_WATER;
+  return RP_OK;
+}
+
 extern "C" int rp_batch_families(rp_batch* b, int* nfam, int* n, int* count) {
   RP_REQUIRE(b && nfam, "rp_batch_families: bad arguments");
   *nfam = (int)b->fam.size();

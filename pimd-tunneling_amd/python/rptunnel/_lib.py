@@ -82,6 +82,7 @@ _SIGS = {
     "rp_batch_get_kick_cursor": [vp, i64p],
     "rp_batch_set_cursor": [vp, i64p, i64p],
     "rp_batch_families": [vp, ip, ip, ip],
+    "rp_batch_family_shape": [vp, C.c_int, ip, ip, ip],
     "rp_batch_ti_readout": [vp, dp, i64p, i64p],
     "rp_batch_set_launch": [vp, C.c_int, C.c_int],
     "rp_batch_set_graphs": [vp, C.c_int],

@@ -116,6 +116,15 @@ class Batch:
         L.check(L.lib().rp_batch_families(self.h, C.byref(nf), L.iptr(n), L.iptr(cnt)))
         return list(zip(n[: nf.value].tolist(), cnt[: nf.value].tolist()))
 
+    def family_shapes(self):
+        """Per family: (n, waves per workgroup, MFMA units per wave, folded transforms)."""
+        out = []
+        for f, (n, _) in enumerate(self.families()):
+            w, u, fo = C.c_int(0), C.c_int(0), C.c_int(0)
+            L.check(L.lib().rp_batch_family_shape(self.h, f, C.byref(w), C.byref(u), C.byref(fo)))
+            out.append((n, w.value, u.value, bool(fo.value)))
+        return out
+
     def set_launch(self, steps_per_launch=0, waves=0):
         L.check(L.lib().rp_batch_set_launch(self.h, steps_per_launch, waves))
 
