@@ -108,7 +108,7 @@ def water_cluster(nmol=6, seed=6, De=0.185, alpha=1.21, re=1.81, kbend=0.16,
         O = np.array([R * np.cos(ph), R * np.sin(ph), 0.45 * (-1) ** m])
         nxt = 2.0 * np.pi * ((m + 1) % nmol) / nmol
         On = np.array([R * np.cos(nxt), R * np.sin(nxt), 0.45 * (-1) ** (m + 1)])
-        e1 = On - O
+        e1 = On - O if nmol > 1 else np.array([1.0, 0.0, 0.0])  # a monomer has no neighbour
         e1 /= np.linalg.norm(e1)
         zax = np.array([0.0, 0.0, 1.0])
         e2 = np.cross(zax, e1)

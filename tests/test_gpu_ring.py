@@ -184,15 +184,31 @@ def test_instanton_batch_is_per_chain_and_caps():
     assert all(np.array_equal(a, b) for a, b in zip(xs, xo))
 
 
-def test_water_param_grad_ring():
-    sp = specs.water_cluster(nmol=2)
+@pytest.mark.parametrize("nmol,ns", [(2, [9, 17]), (6, [128, 64])])
+def test_water_param_grad_ring(nmol, ns):
+    """dU_M/d(qH, De) of every chain: the bead sum of the oracle's sensitivities (pinned by
+    central differences in the parameters), and central differences of the device's own
+    U_M in each parameter (U_M is quadratic in qH and linear in De)."""
+    sp = specs.water_cluster(nmol=nmol)
     pes = Pes(sp)
     op = oracle_pes(sp)
-    ns = [9, 17]
-    ring = Ring(pes, sp.mass, ns, [10.0, 10.0], sp.well1, sp.well2)
+    ring = Ring(pes, sp.mass, ns, [10.0] * len(ns), sp.well1, sp.well2)
     xs = _chains(sp, ns, 9, scale=0.03)
     ring.set_x(xs)
     pg = ring.param_grad()
     for k, n in enumerate(ns):
         want = sum(op.param_grad(xs[k][:, i]) for i in range(n))
         assert rel(pg[k], want) <= 1e-12
+    for j, idx, h in ((0, 7, 1e-3), (1, 0, 1e-3)):
+        ums = []
+        for sgn in (1.0, -1.0):
+            dp = sp.dp.copy()
+            dp[idx] += sgn * h
+            spp = specs.PesSpec(sp.name, sp.kind, sp.ndim, sp.natom, dp, sp.ip, sp.mass,
+                                sp.well1, sp.well2)
+            r2 = Ring(Pes(spp), sp.mass, ns, [10.0] * len(ns), sp.well1, sp.well2)
+            ums.append(r2.UM(xs))
+            r2.close()
+        fd = (ums[0] - ums[1]) / (2.0 * h)
+        for k in range(len(ns)):
+            assert pg[k][j] == pytest.approx(fd[k], rel=1e-8), (j, k, pg[k][j], fd[k])
