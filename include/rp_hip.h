@@ -48,6 +48,23 @@ enum { RP_PES_QUARTIC = 1, RP_PES_COUPLED2D = 2, RP_PES_EVB = 3, RP_PES_WATER = 
 int rp_pes_create(int kind, int ndim, int natom, const double *dp, int ndp, const int *ip,
                   int nip, rp_pes **out);
 int rp_pes_destroy(rp_pes *pes);
+/* A potential that lives on the host: any mcmod_mass plugin with no device twin, e.g.
+ * mcmod_waterdimer.f90:12-103, whose V / Vprime / potforce / Vdoubleprime the reference
+ * calls per bead (verletmodule.f90:573, instantonmod.f90:105, 184).  The library calls
+ * these back with whole chains: x and grad (nbead, ndim, natom) column-major (the layout of
+ * every host buffer here), energy[nbead] as the plugin returns it (its own V - V0);
+ * hess[bead][ndof][ndof] with hess[b][i][k] = d grad_k / d x_i.  A callback returns 0, or
+ * non-zero to stop the calling routine with RP_ERR_ARG.  hessian may be NULL: central
+ * differences of potforce with step 1e-4, as the reference's plugins difference Vprime
+ * (mcmod_waterdimer.f90:83-103).  The plain path (rp_chain: each step's kick calls
+ * potforce once for the chain's beads) and the ring routines (rp_ring) run with it; the
+ * batched engine, whose fused kernel evaluates the potential on the device, refuses it. */
+typedef int (*rp_host_potforce_fn)(void *user, int nbead, const double *x, double *grad,
+                                   double *energy);
+typedef int (*rp_host_hessian_fn)(void *user, int nbead, const double *x, double *hess);
+enum { RP_PES_HOST = 6 };
+int rp_pes_create_host(int ndim, int natom, rp_host_potforce_fn potforce,
+                       rp_host_hessian_fn hessian, void *user, rp_pes **out);
 /* module variable V0 (mcmod_waterdimer.f90:3), set by the driver to V(well1)
  * (pimd_par.f90:166, rpi_ser.f90:95) */
 int rp_pes_set_v0(rp_pes *pes, double v0);
@@ -55,7 +72,7 @@ int rp_pes_set_v0(rp_pes *pes, double v0);
  * x and grad are (nbead, ndim, natom) column-major; energy[nbead] (V - V0); grad may be
  * NULL (V only), energy may be NULL (Vprime only). */
 int rp_pes_potforce(rp_pes *pes, int nbead, const double *x, double *grad, double *energy);
-/* Vdoubleprime (mcmod_waterdimer.f90:143-163) for nbead beads: hess[bead][ndof][ndof] with
+/* Vdoubleprime (mcmod_waterdimer.f90:83-103) for nbead beads: hess[bead][ndof][ndof] with
  * hess[b][i][k] = d grad_k / d x_i, from forward-over-reverse (exact, no finite steps). */
 int rp_pes_hessian(rp_pes *pes, int nbead, const double *x, double *hess);
 /* Adjoint sensitivities of the bead energies to potential parameters (WATER: dV/dqH,
@@ -81,9 +98,14 @@ int rp_chain_get_state(rp_chain *c, double *x, double *p);
 /* copies of the init_nm arrays: T[n*n] row-major, lam[n], beadmass[n*natom] ([mode][atom]),
  * beadvec[ndof*n]; any pointer may be NULL */
 int rp_chain_get_nm(rp_chain *c, double *T, double *lam, double *beadmass, double *beadvec);
-/* Philox stream of this trajectory and its draw cursor (the VSL stream position) */
+/* Philox stream of this trajectory and its draw cursor (the VSL stream position,
+ * verletmodule.f90:363-364); setting the stream also rewinds its Poisson cursor */
 int rp_chain_set_rng(rp_chain *c, uint64_t seed, uint64_t stream, int64_t cursor);
 int rp_chain_get_cursor(rp_chain *c, int64_t *cursor);
+/* position of the stream's Poisson intervals (the Andersen kicks, :202, 230); successive
+ * rp_propagate_nm calls continue it, as the reference's VSL Poisson stream continues */
+int rp_chain_set_kick_cursor(rp_chain *c, int64_t kick_cursor);
+int rp_chain_get_kick_cursor(rp_chain *c, int64_t *kick_cursor);
 /* nmtransform_forward (direction 0) / nmtransform_backward (1) (verletmodule.f90:254-286)
  * applied to ncols n-vectors in[col*n + bead]; shift_dof[col] >= 0 subtracts/adds
  * beadvec(:, dof) as the reference does for positions, -1 for momenta. */
@@ -107,7 +129,7 @@ int rp_init_path(rp_chain *c, double xi, const double *noise);
 /* propagate_pimd_pile (verletmodule.f90:372-416): nmc steps, TI after imin steps,
  * dHdrlimit < 0 disables the re-initialisation.  noise: nnoise draws consumed in order by the
  * O-steps (n*ndof per step) and the re-initialisations (n*ndof each), or NULL for the
- * Philox stream.  dhdr = running sum / (nmc - imin). */
+ * Philox stream.  dhdr = running sum / (nmc - imin), 0 when nmc = imin. */
 int rp_propagate_pile(rp_chain *c, int64_t nmc, int64_t imin, double dhdrlimit,
                       const double *dbdl, double xi, const double *noise, int64_t nnoise,
                       double *dhdr, int64_t *nreinit);
@@ -209,13 +231,19 @@ int rp_ring_hvp(rp_ring *r, const double *v, double *hv);
  * (:183-185) */
 int rp_ring_hessian_band(rp_ring *r, int chain, int singlewell, double *band);
 /* log-determinant of every chain's mass-weighted ring Hessian by FP64 block LU, with
- * the reference's coupling pattern (:203).  logdet_pos[c] = sum of ln(eta) over eta > 0
- * excluding eta_1 when skip_lowest (rpi_ser.f90:355-365), else over all eta > 0
- * (:227-236); nneg[c] = number of negative eigenvalues; eta1[c] = lowest eigenvalue.
- * Returns RP_ERR_CONVERGE for chains where that sum is not determined by det, eta_1 and
- * the inertia (two or more negative eigenvalues). */
+ * the reference's coupling pattern (:203), reduced as detJ's callers reduce the eigenvalues
+ * (ascending, instantonmod.f90:782-827): logdet_pos[c] = sum of ln(eta) over eta > 0,
+ * skipping eta_1 (the lowest) when skip_lowest (lndetJ, rpi_ser.f90:355-365), else over
+ * every eta > 0 (lndetJ0, :227-236); nneg[c] = number of negative eigenvalues (the inertia);
+ * eta1[c] = the lowest eigenvalue.  Negative eigenvalues beyond the one inverse iteration
+ * finds are located by bisection on the inertia, so any spectrum is reduced. */
 int rp_ring_logdet(rp_ring *r, int singlewell, int skip_lowest, double *logdet_pos,
                    int *nneg, double *eta1);
+/* the k lowest eigenvalues of every chain's mass-weighted ring Hessian (the band of
+ * rp_ring_hessian_band), ascending, eigs[chain*k + j], each by bisection on the inertia of
+ * the block LDL^T of H - sigma I to a few ulps: the head of the ascending spectrum DSBEVD
+ * returns in detJ (instantonmod.f90:782-827) */
+int rp_ring_lowest_eigs(rp_ring *r, int singlewell, int k, double *eigs);
 /* instanton (:679-777): L-BFGS-B (m <= 32 corrections, unbounded as nbd=0) on every chain
  * at once, resident on the device (iterates, gradients and correction pairs stay in HBM;
  * gradients from the adjoint).  x_inout as rp_ring_set_x; action[c] = U_M at the optimum;

@@ -280,7 +280,7 @@ void or_pes_Vdoubleprime(const or_pes *pes, const double *x, double *hess) {
   }
   default: {
     /* central finite differences of the gradient, eps = 1e-4, as the reference's
-     * plugins do (mcmod_waterdimer.f90:143-163) */
+     * plugins do (mcmod_waterdimer.f90:83-103) */
     const double e = 1e-4;
     double *xp = (double *)malloc(sizeof(double) * D);
     double *gp = (double *)malloc(sizeof(double) * D), *gm = (double *)malloc(sizeof(double) * D);

@@ -9,7 +9,7 @@
 !   kind 3 EVB       dp = (kappa, Q(D,D), lam(D), cub(D), xa(D)), ip = perm(natom)
 !   kind 5 harmonic  dp = (k(D), c(D))
 ! Vdoubleprime is analytic for kinds 1, 2, 5 and central differences of Vprime with
-! eps = 1e-4 for the EVB, as the reference's plugins do (mcmod_waterdimer.f90:143-163).
+! eps = 1e-4 for the EVB, as the reference's plugins do (mcmod_waterdimer.f90:83-103).
 module mcmod_mass
   implicit none
   double precision::               V0, eps2=1.0d-5
@@ -18,16 +18,27 @@ module mcmod_mass
   integer::                        n, ndim, ndof, natom, xunit, totdof
   logical::                        potforcepresent
   character(len=20)::              basename
-  integer::                        pes_kind
+  integer::                        pes_kind = 0
   double precision, allocatable::  pes_dp(:)
   integer, allocatable::           pes_ip(:)
 
 contains
 
+  ! A driver that sets no potential (the reference's own drivers only call V_init) reads it
+  ! from rp_pes.dat, in the format of the device plugin mcmod_rp.f90:
+  !     kind ndim natom ndp nip / dp(1:ndp) / ip(1:nip), ip 0-based atom indices
   subroutine V_init(iproc)
     integer, intent(in):: iproc
+    integer:: nd, na, ndp, nip, u
     potforcepresent=.true.
     V0=0.0d0
+    if (pes_kind /= 0) return
+    open(newunit=u, file='rp_pes.dat', status='old', action='read')
+    read(u,*) pes_kind, nd, na, ndp, nip
+    allocate(pes_dp(ndp), pes_ip(max(nip,1)))
+    read(u,*) pes_dp
+    if (nip > 0) read(u,*) pes_ip(1:nip)
+    close(u)
   end subroutine V_init
 
   ! energy and gradient in dof order, before the V0 shift

@@ -222,6 +222,9 @@ extern "C" int rp_batch_create(rp_pes* pes, const double* mass, int npoly,
              "rp_batch_create: thermostat must be 1 (Andersen) or 2 (Langevin)");
   RP_REQUIRE(tau > 0 && dt > 0, "rp_batch_create: tau and dt must be positive");
   RP_REQUIRE(thermostat != RP_THERMO_ANDERSEN || noutput > 0, "rp_batch_create: noutput <= 0");
+  RP_REQUIRE(pes->kind != PES_HOST,
+             "rp_batch_create: the fused kernel evaluates the potential on the device; a "
+             "potential that lives on the host runs through the plain path (rp_nm_setup)");
   int rc = ensure_init();
   if (rc) return rc;
   rp_batch* b = new rp_batch();

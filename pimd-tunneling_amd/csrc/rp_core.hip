@@ -127,9 +127,76 @@ __global__ void __launch_bounds__(128) water_param_grad_kernel(PesDev P, int nbe
   out[2 * b + 1] = pg[1];
 }
 
+// p -= h * g elementwise (the kick of a potential whose gradient g came from the host)
+__global__ void __launch_bounds__(256) kick_grad_kernel(int64_t count, const double* __restrict__ g,
+                                                        double* __restrict__ p, double h,
+                                                        int* __restrict__ nan_flag) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  const double np = p[e] - g[e] * h;
+  p[e] = np;
+  if (np != np) atomicOr(nan_flag, 1);
+}
+
+int host_pes_eval(const rp_pes* pes, int nbead, int ld, const double* x, double* grad,
+                  double* energy, hipStream_t s) {
+  if (nbead <= 0) return RP_OK;
+  const int D = pes->ndof;
+  const size_t nx = (size_t)nbead * D;
+  pes->hx.resize(nx);
+  pes->hg.resize(nx);
+  pes->he.resize(nbead);
+  // [dof][bead] of stride ld on the device <-> (nbead, ndim, natom) column-major on the host
+  RP_HIP(hipMemcpy2DAsync(pes->hx.data(), nbead * sizeof(double), x, ld * sizeof(double),
+                          nbead * sizeof(double), D, hipMemcpyDeviceToHost, s));
+  RP_HIP(hipStreamSynchronize(s));
+  if (pes->host_pf(pes->host_user, nbead, pes->hx.data(), pes->hg.data(), pes->he.data()) != 0)
+    return set_error(RP_ERR_ARG, "the host potential's potforce callback reported an error");
+  if (grad)
+    RP_HIP(hipMemcpy2DAsync(grad, ld * sizeof(double), pes->hg.data(), nbead * sizeof(double),
+                            nbead * sizeof(double), D, hipMemcpyHostToDevice, s));
+  if (energy) {
+    for (double& e : pes->he) e -= pes->v0;
+    RP_HIP(hipMemcpyAsync(energy, pes->he.data(), nbead * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  RP_HIP(hipStreamSynchronize(s));  // the staging vectors are reused by the next call
+  return RP_OK;
+}
+
+int host_pes_hessian(const rp_pes* pes, int nbead, const double* x, double* hess) {
+  const int D = pes->ndof;
+  if (nbead <= 0) return RP_OK;
+  if (pes->host_hs) {
+    if (pes->host_hs(pes->host_user, nbead, x, hess) != 0)
+      return set_error(RP_ERR_ARG, "the host potential's hessian callback reported an error");
+    return RP_OK;
+  }
+  // central differences of the gradient, step 1e-4 (mcmod_waterdimer.f90:83-103); a bead's
+  // gradient depends on that bead alone, so dof i of every bead moves at once
+  const double eps = 1e-4;
+  const size_t nx = (size_t)nbead * D;
+  std::vector<double> xp(x, x + nx), gp(nx), gm(nx), e(nbead);
+  for (int i = 0; i < D; ++i) {
+    for (int b = 0; b < nbead; ++b) xp[(size_t)i * nbead + b] = x[(size_t)i * nbead + b] + eps;
+    if (pes->host_pf(pes->host_user, nbead, xp.data(), gp.data(), e.data()) != 0)
+      return set_error(RP_ERR_ARG, "the host potential's potforce callback reported an error");
+    for (int b = 0; b < nbead; ++b) xp[(size_t)i * nbead + b] = x[(size_t)i * nbead + b] - eps;
+    if (pes->host_pf(pes->host_user, nbead, xp.data(), gm.data(), e.data()) != 0)
+      return set_error(RP_ERR_ARG, "the host potential's potforce callback reported an error");
+    for (int b = 0; b < nbead; ++b) {
+      xp[(size_t)i * nbead + b] = x[(size_t)i * nbead + b];
+      for (int k = 0; k < D; ++k)
+        hess[((size_t)b * D + i) * D + k] =
+            (gp[(size_t)k * nbead + b] - gm[(size_t)k * nbead + b]) / (2.0 * eps);
+    }
+  }
+  return RP_OK;
+}
+
 int launch_pes_eval(const rp_pes* pes, int nbead, int ld, const double* x, double* grad,
                     double* energy, hipStream_t s) {
   if (nbead <= 0) return RP_OK;
+  if (pes->kind == PES_HOST) return host_pes_eval(pes, nbead, ld, x, grad, energy, s);
   PesDev P = pes->dev();
   return dispatch_pes(pes, [&]<int K, int M>() -> int {
     pes_eval_kernel<K, M><<<(nbead + 127) / 128, 128, 0, s>>>(P, nbead, ld, x, grad, energy);
@@ -153,6 +220,19 @@ int launch_pes_kick(const rp_pes* pes, int n, int nchain, const double* x, doubl
                     int* nan_flag, hipStream_t s) {
   int tot = n * nchain;
   if (tot <= 0) return RP_OK;
+  if (pes->kind == PES_HOST) {
+    // the plugin's gradient for every bead of each chain, then the kick on the device
+    const int64_t count = (int64_t)tot * pes->ndof;
+    int rc;
+    if (pes->d_hg.n < (size_t)count && (rc = pes->d_hg.alloc(count))) return rc;
+    for (int c = 0; c < nchain; ++c) {
+      const size_t o = (size_t)c * pes->ndof * n;
+      if ((rc = host_pes_eval(pes, n, n, x + o, pes->d_hg.p + o, nullptr, s))) return rc;
+    }
+    kick_grad_kernel<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(count, pes->d_hg.p, p, h, nan_flag);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  }
   PesDev P = pes->dev();
   return dispatch_pes(pes, [&]<int K, int M>() -> int {
     pes_kick_kernel<K, M><<<(tot + 127) / 128, 128, 0, s>>>(P, n, nchain, x, p, h, nan_flag);
@@ -289,6 +369,31 @@ extern "C" int rp_pes_create(int kind, int ndim, int natom, const double* dp, in
   return RP_OK;
 }
 
+extern "C" int rp_pes_create_host(int ndim, int natom, rp_host_potforce_fn potforce,
+                                  rp_host_hessian_fn hessian, void* user, rp_pes** out) {
+  RP_REQUIRE(out != nullptr, "rp_pes_create_host: out is NULL");
+  *out = nullptr;
+  RP_REQUIRE(ndim > 0 && natom > 0, "rp_pes_create_host: ndim=%d natom=%d", ndim, natom);
+  RP_REQUIRE(potforce != nullptr, "rp_pes_create_host: potforce callback is NULL");
+  int rc = ensure_init();
+  if (rc) return rc;
+  rp_pes* P = new rp_pes();
+  P->kind = PES_HOST;
+  P->ndim = ndim;
+  P->natom = natom;
+  P->ndof = ndim * natom;
+  P->v0 = 0.0;  // the plugin subtracts its own V0
+  P->host_pf = potforce;
+  P->host_hs = hessian;
+  P->host_user = user;
+  if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete P;
+    return set_error(RP_ERR_HIP, "rp_pes_create_host: stream creation failed");
+  }
+  *out = P;
+  return RP_OK;
+}
+
 extern "C" int rp_pes_destroy(rp_pes* pes) {
   if (!pes) return RP_OK;
   (void)hipStreamSynchronize(pes->stream);
@@ -297,6 +402,7 @@ extern "C" int rp_pes_destroy(rp_pes* pes) {
   pes->d_x.free();
   pes->d_g.free();
   pes->d_e.free();
+  pes->d_hg.free();
   (void)hipStreamDestroy(pes->stream);
   delete pes;
   return RP_OK;
@@ -322,6 +428,15 @@ extern "C" int rp_pes_potforce(rp_pes* pes, int nbead, const double* x, double* 
   if (nbead == 0) return RP_OK;
   int rc;
   const size_t nx = (size_t)nbead * pes->ndof;
+  if (pes->kind == PES_HOST) {  // the caller's own plugin, called directly
+    std::vector<double> g(nx), e(nbead);
+    if (pes->host_pf(pes->host_user, nbead, x, g.data(), e.data()) != 0)
+      return set_error(RP_ERR_ARG, "the host potential's potforce callback reported an error");
+    if (grad) memcpy(grad, g.data(), nx * sizeof(double));
+    if (energy)
+      for (int b = 0; b < nbead; ++b) energy[b] = e[b] - pes->v0;
+    return RP_OK;
+  }
   if ((rc = ensure_init()) || (rc = pes_scratch(pes, nx, nx, nbead))) return rc;
   if ((rc = pes->d_x.upload(x, nx, pes->stream))) return rc;
   if ((rc = launch_pes_eval(pes, nbead, nbead, pes->d_x.p, grad ? pes->d_g.p : nullptr,
@@ -336,6 +451,7 @@ extern "C" int rp_pes_potforce(rp_pes* pes, int nbead, const double* x, double* 
 extern "C" int rp_pes_hessian(rp_pes* pes, int nbead, const double* x, double* hess) {
   RP_REQUIRE(pes && x && hess && nbead >= 0, "rp_pes_hessian: bad arguments");
   if (nbead == 0) return RP_OK;
+  if (pes->kind == PES_HOST) return host_pes_hessian(pes, nbead, x, hess);
   RP_REQUIRE(pes->ndof <= 64, "rp_pes_hessian: ndof %d > 64", pes->ndof);
   int rc;
   const size_t nx = (size_t)nbead * pes->ndof, nh = nx * pes->ndof;

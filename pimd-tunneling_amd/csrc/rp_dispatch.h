@@ -47,6 +47,10 @@ int dispatch_pes(const rp_pes* pes, F&& f) {
       if (D == 27) return f.template operator()<PES_EVB, 27>();
       if (D <= 32) return f.template operator()<PES_EVB, 32>();
       break;
+    case PES_HOST:
+      return set_error(RP_ERR_ARG, "a potential that lives on the host has no device kernel "
+                                   "on this path (the batched engine and the parameter "
+                                   "sensitivities need a device potential)");
     case PES_WATER:
       if (pes->ndim == 3 && pes->natom == 3) return f.template operator()<PES_WATER, 9>();
       if (pes->ndim == 3 && pes->natom == 6) return f.template operator()<PES_WATER, 18>();

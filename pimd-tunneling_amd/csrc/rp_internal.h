@@ -71,6 +71,13 @@ struct rp_pes {
   rp::DevBuf<int> d_ip;
   hipStream_t stream;
   rp::DevBuf<double> d_x, d_g, d_e;  // scratch for the per-call entry points
+  // a potential that lives on the host (kind PES_HOST): the plugin's callbacks, the host
+  // staging of their arguments and the device copy of their gradients
+  rp_host_potforce_fn host_pf = nullptr;
+  rp_host_hessian_fn host_hs = nullptr;
+  void* host_user = nullptr;
+  mutable std::vector<double> hx, hg, he;
+  mutable rp::DevBuf<double> d_hg;
   rp::PesDev dev() const {
     rp::PesDev P;
     P.kind = kind;
@@ -96,6 +103,14 @@ int launch_pes_eval(const rp_pes* pes, int nbead, int ld, const double* x, doubl
 // Hessian-vector products of nbead beads: hv = H(x) v, layout as launch_pes_eval
 int launch_pes_hvp(const rp_pes* pes, int nbead, int ld, const double* x, const double* v,
                    double* hv, hipStream_t s);
+// A potential that lives on the host: energies and gradients of nbead beads from its
+// potforce callback.  x and grad are device buffers in [dof][bead] layout of stride ld (grad
+// may be null), energy a device array of nbead (may be null).  Synchronises s.
+int host_pes_eval(const rp_pes* pes, int nbead, int ld, const double* x, double* grad,
+                  double* energy, hipStream_t s);
+// Host potential: hess[b][i][k] = d grad_k / d x_i of nbead beads at the host chain x
+// ((nbead, ndim, natom) column-major), from the hessian callback or central differences.
+int host_pes_hessian(const rp_pes* pes, int nbead, const double* x, double* hess);
 // MAXD bucket a potential needs (16, 32, 64) or 0 when unsupported
 int pes_maxd(const rp_pes* pes);
 // row stride of the device EVB layout (the MAXD bucket dispatch_pes picks)

@@ -19,7 +19,8 @@
 
 namespace rp {
 
-enum { PES_QUARTIC = 1, PES_COUPLED2D = 2, PES_EVB = 3, PES_WATER = 4, PES_HARMONIC = 5 };
+enum { PES_QUARTIC = 1, PES_COUPLED2D = 2, PES_EVB = 3, PES_WATER = 4, PES_HARMONIC = 5,
+       PES_HOST = 6 /* callbacks into a host plugin; no device form */ };
 
 struct PesDev {
   int kind, ndim, natom, ndof;

@@ -33,7 +33,8 @@ struct rp_chain {
   int npath = 0;
   DevBuf<double> d_lampath, d_path, d_y2;
   uint64_t seed = 0, stream = 0;
-  int64_t cursor = 0;
+  int64_t cursor = 0;   // next normal of the stream
+  int64_t kcursor = 0;  // next Poisson interval of the stream (Andersen)
 };
 
 // ---------------------------------------------------------------- kernels
@@ -332,12 +333,25 @@ extern "C" int rp_chain_set_rng(rp_chain* c, uint64_t seed, uint64_t stream, int
   c->seed = seed;
   c->stream = stream;
   c->cursor = cursor;
+  c->kcursor = 0;
   return RP_OK;
 }
 
 extern "C" int rp_chain_get_cursor(rp_chain* c, int64_t* cursor) {
   RP_REQUIRE(c && cursor, "rp_chain_get_cursor: NULL argument");
   *cursor = c->cursor;
+  return RP_OK;
+}
+
+extern "C" int rp_chain_set_kick_cursor(rp_chain* c, int64_t kick_cursor) {
+  RP_REQUIRE(c && kick_cursor >= 0, "rp_chain_set_kick_cursor: bad arguments");
+  c->kcursor = kick_cursor;
+  return RP_OK;
+}
+
+extern "C" int rp_chain_get_kick_cursor(rp_chain* c, int64_t* kick_cursor) {
+  RP_REQUIRE(c && kick_cursor, "rp_chain_get_kick_cursor: NULL argument");
+  *kick_cursor = c->kcursor;
   return RP_OK;
 }
 
@@ -454,7 +468,7 @@ extern "C" int rp_init_path(rp_chain* c, double xi, const double* noise) {
 extern "C" int rp_propagate_pile(rp_chain* c, int64_t nmc, int64_t imin, double dhdrlimit,
                                  const double* dbdl, double xi, const double* noise,
                                  int64_t nnoise, double* dhdr, int64_t* nreinit) {
-  RP_REQUIRE(c && dbdl && dhdr && nmc > imin && imin >= 0, "rp_propagate_pile: bad arguments");
+  RP_REQUIRE(c && dbdl && dhdr && nmc >= imin && imin >= 0, "rp_propagate_pile: bad arguments");
   const int n = c->n, D = c->ndof;
   const size_t nd = (size_t)n * D;
   int rc;
@@ -500,7 +514,7 @@ extern "C" int rp_propagate_pile(rp_chain* c, int64_t nmc, int64_t imin, double 
   double sum = 0.0;
   if ((rc = c->d_ti.download(&sum, 1, c->s))) return rc;
   if ((rc = nan_status(c))) return rc;
-  *dhdr = sum / (double)(nmc - imin);
+  *dhdr = nmc > imin ? sum / (double)(nmc - imin) : 0.0;
   if (nreinit) *nreinit = reinit;
   return RP_OK;
 }
@@ -508,20 +522,20 @@ extern "C" int rp_propagate_pile(rp_chain* c, int64_t nmc, int64_t imin, double 
 extern "C" int rp_propagate_nm(rp_chain* c, int64_t nmc, int64_t imin, int64_t noutput,
                                const double* dbdl, const double* noise, int64_t nnoise,
                                const int32_t* kicks, int64_t nkicks, double* dhdr) {
-  RP_REQUIRE(c && dbdl && dhdr && nmc > imin && imin >= 0 && noutput > 0,
+  RP_REQUIRE(c && dbdl && dhdr && nmc >= imin && imin >= 0 && noutput > 0,
              "rp_propagate_nm: bad arguments");
   const int n = c->n, D = c->ndof;
   const size_t nd = (size_t)n * D;
   int rc;
   if (noise && (rc = upload_noise(c, noise, (size_t)nnoise))) return rc;
   size_t ncur = 0;
-  int64_t kcur = 0, pcur = 0;  // Philox Poisson draws use their own index
+  int64_t kcur = 0;  // injected intervals restart per call; the Philox ones continue
   auto next_kick = [&](int32_t* k) -> int {
     if (kicks) {
       if (kcur >= nkicks) return set_error(RP_ERR_NOISE, "injected Poisson draws exhausted");
       *k = kicks[kcur++];
     } else {
-      *k = philox_poisson(c->seed, c->stream, pcur++, (double)noutput);
+      *k = philox_poisson(c->seed, c->stream, c->kcursor++, (double)noutput);
     }
     return RP_OK;
   };
@@ -555,6 +569,6 @@ extern "C" int rp_propagate_nm(rp_chain* c, int64_t nmc, int64_t imin, int64_t n
   double sum = 0.0;
   if ((rc = c->d_ti.download(&sum, 1, c->s))) return rc;
   if ((rc = nan_status(c))) return rc;
-  *dhdr = sum / (double)(nmc - imin);
+  *dhdr = nmc > imin ? sum / (double)(nmc - imin) : 0.0;
   return RP_OK;
 }

@@ -324,6 +324,79 @@ __global__ void __launch_bounds__(128) ring_hblock_kernel(RingArgs A, const doub
     if (k < D) blk[k] = gs[k].d;
 }
 
+// U_M and its gradient for a potential that lives on the host: the bead energies pe[gbead]
+// and the potential's gradients (already in grad) come from the plugin; this adds the
+// springs exactly as ring_um_kernel does and sums each chain's energy.
+__global__ void __launch_bounds__(kRingTile) ring_spring_kernel(RingArgs A, const double* __restrict__ x,
+                                                                const double* __restrict__ pe,
+                                                                double* __restrict__ grad,
+                                                                double* __restrict__ energy) {
+  __shared__ double red[kRingTile];
+  const int c = blockIdx.y;
+  const int n = A.n[c], D = A.D;
+  const int i = blockIdx.x * kRingTile + threadIdx.x;
+  if (blockIdx.x * kRingTile >= n) return;  // the same for the whole workgroup
+  const double bn2 = A.betan[c] * A.betan[c];
+  const double* xc = x + A.off[c];
+  const double* a = A.ab + (size_t)c * 2 * D;
+  const double* b = a + D;
+  double e = 0.0;
+  if (i < n) {
+    e = pe[A.boff[c] + i];
+    for (int d = 0; d < D; ++d) {
+      const double m = A.massd[d];
+      const double* xd = xc + (size_t)d * n;
+      const double xi = xd[i];
+      if (i < n - 1) e += (0.5 * m / bn2) * (xd[i + 1] - xi) * (xd[i + 1] - xi);
+      double sg;
+      if (i == 0)
+        sg = A.fixedends ? m * (2.0 * xi - a[d] - xd[1]) / bn2 : m * (xi - xd[1]) / bn2;
+      else if (i == n - 1)
+        sg = A.fixedends ? m * (2.0 * xi - xd[n - 2] - b[d]) / bn2 : m * (xi - xd[n - 2]) / bn2;
+      else
+        sg = m * (2.0 * xi - xd[i - 1] - xd[i + 1]) / bn2;
+      if (A.fixedends && i == 0) e += (0.5 * m / bn2) * (xi - a[d]) * (xi - a[d]);
+      if (A.fixedends && i == n - 1) e += (0.5 * m / bn2) * (b[d] - xi) * (b[d] - xi);
+      if (grad) grad[A.off[c] + (size_t)d * n + i] += sg;
+    }
+  }
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int st = kRingTile / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(&energy[c], red[0]);
+}
+
+// Hessian-vector product of U_M from stored potential Hessian blocks (a potential that lives
+// on the host): the springs of ring_hvp_kernel plus sum_i blocks[gbead][i][d] v_i.
+__global__ void __launch_bounds__(kRingTile) ring_hvp_blocks_kernel(RingArgs A,
+                                                                    const double* __restrict__ blocks,
+                                                                    const double* __restrict__ v,
+                                                                    double* __restrict__ hv) {
+  const int c = blockIdx.y;
+  const int n = A.n[c], D = A.D;
+  const int i = blockIdx.x * kRingTile + threadIdx.x;
+  if (i >= n) return;
+  const double bn2 = A.betan[c] * A.betan[c];
+  const int64_t o = A.off[c];
+  const double* blk = blocks + (size_t)(A.boff[c] + i) * D * D;
+  for (int d = 0; d < D; ++d) {
+    const double* vd = v + o + (size_t)d * n;
+    double s;
+    if (i == 0)
+      s = A.fixedends ? 2.0 * vd[0] - (n > 1 ? vd[1] : 0.0) : vd[0] - (n > 1 ? vd[1] : 0.0);
+    else if (i == n - 1)
+      s = A.fixedends ? 2.0 * vd[i] - vd[i - 1] : vd[i] - vd[i - 1];
+    else
+      s = 2.0 * vd[i] - vd[i - 1] - vd[i + 1];
+    double pot = 0.0;
+    for (int k = 0; k < D; ++k) pot += blk[(size_t)k * D + d] * v[o + (size_t)k * n + i];
+    hv[o + (size_t)d * n + i] = A.massd[d] * s / bn2 + pot;
+  }
+}
+
 // UMhessian's band (instantonmod.f90:155-217) from the blocks of one chain, in the
 // reference's array answer(ndof+1, n*ndof) (LAPACK lower band storage, column-major):
 // band[col*(D+1) + r] = A(col + r, col), r = 0..D.  The inter-bead coupling
@@ -420,15 +493,18 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
     int D, const int* __restrict__ nn, const int64_t* __restrict__ boff,
     const double* __restrict__ betan, const double* __restrict__ massd,
     const double* __restrict__ blocks, const double* __restrict__ shift, double* __restrict__ sinv,
-    double* __restrict__ logdet, int* __restrict__ nneg) {
+    double* __restrict__ logdet, int* __restrict__ nneg, const int* __restrict__ chain_of) {
   // every LDS element t + T q exists, and so does every r D + k <= t + T q + D - 1 read for
   // it; the elements beyond D*D hold values nobody reads back, which keeps every load and
   // store unconditional (no branch, so the loads of the next block stay in flight)
   constexpr int kTile = T * P + kMaxD > kMaxD * (kMaxD + 1) ? T * P + kMaxD : kMaxD * (kMaxD + 1);
   __shared__ double Sb[2][kTile];
-  const int c = blockIdx.x, n = nn[c], tid = threadIdx.x, DD = D * D;
+  // one task per workgroup: chain c (task t of a chain_of map, else chain t) factored at
+  // the shift of task t; outputs per task
+  const int t = blockIdx.x, c = chain_of ? chain_of[t] : t;
+  const int n = nn[c], tid = threadIdx.x, DD = D * D;
   const double bn2 = betan[c] * betan[c], c2 = 1.0 / (bn2 * bn2);
-  const double sig = shift ? shift[c] : 0.0;  // factor H - sig I
+  const double sig = shift ? shift[t] : 0.0;  // factor H - sig I
   int er[P], ec[P], eg[P];
   double wgt[P], diag[P], nxt[P];
 #pragma unroll
@@ -492,15 +568,57 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
       lg_tot += l;
       neg_tot += ng;
     }
-    double* out = sinv + (size_t)(boff[c] + i) * DD;
-    const double* Minv = Sb[cur];
+    if (sinv) {  // kept for inverse iteration; inertia-only calls pass null
+      double* out = sinv + (size_t)(boff[c] + i) * DD;
+      const double* Minv = Sb[cur];
 #pragma unroll
-    for (int q = 0; q < P; ++q)
-      if (tid + T * q < DD) out[tid + T * q] = Minv[tid + T * q];
+      for (int q = 0; q < P; ++q)
+        if (tid + T * q < DD) out[tid + T * q] = Minv[tid + T * q];
+    }
   }
   if (tid == 0) {
-    logdet[c] = lg_tot;
-    nneg[c] = neg_tot;
+    logdet[t] = lg_tot;
+    nneg[t] = neg_tot;
+  }
+}
+
+// Gershgorin bounds of every chain's mass-weighted ring Hessian: over rows (bead i, dof r),
+// the minimum and maximum of A_i(r, r) -/+ [sum_{c != r} |A_i(r, c)| + (coupled neighbours of
+// bead i)/bn^2].  lohi[2c] = lower, lohi[2c+1] = upper.
+__global__ void __launch_bounds__(kRingThreads) ring_gersh_kernel(
+    int D, const int* __restrict__ nn, const int64_t* __restrict__ boff,
+    const double* __restrict__ betan, const double* __restrict__ massd,
+    const double* __restrict__ blocks, double* __restrict__ lohi) {
+  __shared__ double rlo[kRingThreads], rhi[kRingThreads];
+  const int c = blockIdx.x, n = nn[c];
+  const double ibn2 = 1.0 / (betan[c] * betan[c]);
+  const double* base = blocks + (size_t)boff[c] * D * D;
+  double vlo = INFINITY, vhi = -INFINITY;
+  for (int64_t e = threadIdx.x; e < (int64_t)n * D; e += kRingThreads) {
+    const int i = (int)(e / D), r = (int)(e - (int64_t)i * D);
+    const double* row = base + ((size_t)i * D + r) * D;
+    double rad = 0.0;
+    for (int k = 0; k < D; ++k)
+      if (k != r) rad += fabs(row[k]) / sqrt(massd[r] * massd[k]);
+    const int nb = (i >= 2 ? 1 : 0) + (i >= 1 && i + 1 < n ? 1 : 0);
+    rad += nb * ibn2;
+    const double ctr = 2.0 * ibn2 + row[r] / massd[r];
+    vlo = fmin(vlo, ctr - rad);
+    vhi = fmax(vhi, ctr + rad);
+  }
+  rlo[threadIdx.x] = vlo;
+  rhi[threadIdx.x] = vhi;
+  __syncthreads();
+  for (int s = kRingThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      rlo[threadIdx.x] = fmin(rlo[threadIdx.x], rlo[threadIdx.x + s]);
+      rhi[threadIdx.x] = fmax(rhi[threadIdx.x], rhi[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    lohi[2 * c] = rlo[0];
+    lohi[2 * c + 1] = rhi[0];
   }
 }
 
@@ -619,6 +737,9 @@ struct rp_ring {
       d_eta;
   DevBuf<int> d_n, d_nneg;
   DevBuf<int64_t> d_off, d_boff;
+  // a potential that lives on the host: bead energies on the device, host staging
+  DevBuf<double> d_pe;
+  std::vector<double> hx, hg, he;
 };
 
 static RingArgs ring_args(rp_ring* r) {
@@ -639,10 +760,40 @@ static RingArgs ring_args(rp_ring* r) {
 }
 
 namespace rp {
+// The potential of every bead from a host plugin: one copy of the chains to the host, the
+// plugin's potforce per chain, the gradients and bead energies back (synchronous).
+static int ring_host_potential(rp_ring* r, bool want_grad) {
+  int rc;
+  if (r->d_pe.n < (size_t)r->nbeads && (rc = r->d_pe.alloc(r->nbeads))) return rc;
+  r->hx.resize(r->tot);
+  r->hg.resize(r->tot);
+  r->he.resize(r->nbeads);
+  if ((rc = r->d_x.download(r->hx.data(), r->tot, r->s))) return rc;
+  RP_HIP(hipStreamSynchronize(r->s));
+  const rp_pes* P = r->pes;
+  for (int c = 0; c < r->nchain; ++c)
+    if (P->host_pf(P->host_user, r->n[c], r->hx.data() + r->off[c], r->hg.data() + r->off[c],
+                   r->he.data() + r->boff[c]) != 0)
+      return set_error(RP_ERR_ARG, "the host potential's potforce callback reported an error");
+  for (double& e : r->he) e -= P->v0;
+  if (want_grad && (rc = r->d_g.upload(r->hg.data(), r->tot, r->s))) return rc;
+  if ((rc = r->d_pe.upload(r->he.data(), r->nbeads, r->s))) return rc;
+  RP_HIP(hipStreamSynchronize(r->s));
+  return RP_OK;
+}
+
 int ring_eval(rp_ring* r, bool want_grad) {
   RingArgs a = ring_args(r);
   RP_HIP(hipMemsetAsync(r->d_e.p, 0, sizeof(double) * r->nchain, r->s));
   const dim3 grid((unsigned)((r->nmax + kRingTile - 1) / kRingTile), (unsigned)r->nchain);
+  if (r->pes->kind == PES_HOST) {
+    int rc = ring_host_potential(r, want_grad);
+    if (rc) return rc;
+    ring_spring_kernel<<<grid, kRingTile, 0, r->s>>>(a, r->d_x.p, r->d_pe.p,
+                                                     want_grad ? r->d_g.p : nullptr, r->d_e.p);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  }
   return dispatch_pes(r->pes, [&]<int K, int M>() -> int {
     if constexpr (K == PES_EVB) {  // the EVB's two GEMMs per bead tile on the matrix cores
       constexpr int TB = kUmWaves * 16;
@@ -738,7 +889,8 @@ extern "C" int rp_ring_destroy(rp_ring* r) {
   if (!r) return RP_OK;
   if (r->s) (void)hipStreamSynchronize(r->s);
   for (DevBuf<double>* b : {&r->d_x, &r->d_g, &r->d_v, &r->d_hv, &r->d_e, &r->d_ab, &r->d_betan,
-                            &r->d_massd, &r->d_blocks, &r->d_sinv, &r->d_work, &r->d_ld, &r->d_eta})
+                            &r->d_massd, &r->d_blocks, &r->d_sinv, &r->d_work, &r->d_ld, &r->d_eta,
+                            &r->d_pe})
     b->free();
   r->d_n.free();
   r->d_nneg.free();
@@ -772,11 +924,22 @@ extern "C" int rp_ring_um(rp_ring* r, double* energy, double* grad) {
   return ring_download(r, energy, grad);
 }
 
+static int ring_blocks(rp_ring* r, int singlewell);
+
 extern "C" int rp_ring_hvp(rp_ring* r, const double* v, double* hv) {
   RP_REQUIRE(r && v && hv, "rp_ring_hvp: bad arguments");
   int rc;
   if ((rc = r->d_v.upload(v, r->tot, r->s))) return rc;
   RingArgs a = ring_args(r);
+  if (r->pes->kind == PES_HOST) {  // the plugin's Hessian blocks, then the block product
+    if ((rc = ring_blocks(r, 0))) return rc;
+    const dim3 grid((unsigned)((r->nmax + kRingTile - 1) / kRingTile), (unsigned)r->nchain);
+    ring_hvp_blocks_kernel<<<grid, kRingTile, 0, r->s>>>(a, r->d_blocks.p, r->d_v.p, r->d_hv.p);
+    RP_HIP(hipGetLastError());
+    if ((rc = r->d_hv.download(hv, r->tot, r->s))) return rc;
+    RP_HIP(hipStreamSynchronize(r->s));
+    return RP_OK;
+  }
   rc = dispatch_pes(r->pes, [&]<int K, int M>() -> int {
     const dim3 grid((unsigned)((r->nmax + kRingTile - 1) / kRingTile), (unsigned)r->nchain);
     ring_hvp_kernel<K, M><<<grid, kRingTile, 0, r->s>>>(a, r->d_x.p, r->d_v.p, r->d_hv.p);
@@ -793,6 +956,28 @@ static int ring_blocks(rp_ring* r, int singlewell) {
   int rc;
   const size_t nb = (size_t)r->nbeads * r->D * r->D;
   if (r->d_blocks.n < nb && (rc = r->d_blocks.alloc(nb))) return rc;
+  if (r->pes->kind == PES_HOST) {
+    // the plugin's Vdoubleprime (or differenced gradients) per bead; singlewell reuses
+    // bead 1's block for every bead (instantonmod.f90:183-185)
+    const int D = r->D;
+    std::vector<double> hx(r->tot), hb(nb), x1(D);
+    if ((rc = r->d_x.download(hx.data(), r->tot, r->s))) return rc;
+    RP_HIP(hipStreamSynchronize(r->s));
+    for (int c = 0; c < r->nchain; ++c) {
+      const int n = r->n[c];
+      double* out = hb.data() + (size_t)r->boff[c] * D * D;
+      if (singlewell) {
+        for (int d = 0; d < D; ++d) x1[d] = hx[r->off[c] + (size_t)d * n];
+        if ((rc = host_pes_hessian(r->pes, 1, x1.data(), out))) return rc;
+        for (int i = 1; i < n; ++i) memcpy(out + (size_t)i * D * D, out, (size_t)D * D * sizeof(double));
+      } else if ((rc = host_pes_hessian(r->pes, n, hx.data() + r->off[c], out))) {
+        return rc;
+      }
+    }
+    if ((rc = r->d_blocks.upload(hb.data(), nb, r->s))) return rc;
+    RP_HIP(hipStreamSynchronize(r->s));
+    return RP_OK;
+  }
   RingArgs a = ring_args(r);
   int maxn = 0;
   for (int c = 0; c < r->nchain; ++c) maxn = std::max(maxn, r->n[c]);
@@ -845,7 +1030,7 @@ int ring_lowest_eigen(rp_ring* r, double* eta) {
     if ((rc = with_logdet_kernel(r->D, [&](auto kern, int nt) -> int {
           kern<<<r->nchain, nt, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p, r->d_massd.p,
                                           r->d_blocks.p, d_shift.p, r->d_sinv.p, r->d_ld.p,
-                                          r->d_nneg.p);
+                                          r->d_nneg.p, nullptr);
           RP_HIP(hipGetLastError());
           return RP_OK;
         })))
@@ -862,6 +1047,125 @@ int ring_lowest_eigen(rp_ring* r, double* eta) {
 }
 }  // namespace rp
 
+namespace rp {
+// Gershgorin bounds of every chain's band matrix (blocks already formed): lohi[2c], lohi[2c+1].
+static int ring_gersh(rp_ring* r, std::vector<double>& lohi) {
+  DevBuf<double> d_lohi;
+  int rc;
+  if ((rc = d_lohi.alloc(2 * (size_t)r->nchain))) return rc;
+  ring_gersh_kernel<<<r->nchain, kRingThreads, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p,
+                                                          r->d_massd.p, r->d_blocks.p, d_lohi.p);
+  lohi.resize(2 * (size_t)r->nchain);
+  if (hipGetLastError() != hipSuccess) rc = set_error(RP_ERR_HIP, "Gershgorin kernel launch");
+  if (!rc) rc = d_lohi.download(lohi.data(), lohi.size(), r->s);
+  if (!rc && hipStreamSynchronize(r->s) != hipSuccess) rc = set_error(RP_ERR_HIP, "Gershgorin bounds");
+  d_lohi.free();
+  return rc;
+}
+
+// Eigenvalues by bisection on Sylvester's inertia: the number of eigenvalues below sigma is
+// the number of negative pivots of the block LDL^T of H - sigma I.  Task t brackets the
+// j[t]-th smallest eigenvalue (0-based) of chain c[t] in [a[t], b[t]], with at most j[t]
+// eigenvalues below a[t] and more than j[t] below b[t]; one launch factors every task at its
+// midpoint.  eig[t] gets the midpoint of the final bracket, within a few ulps of it.
+static int ring_bisect(rp_ring* r, const std::vector<int>& tchain, const std::vector<int>& tj,
+                       std::vector<double>& a, std::vector<double>& b, std::vector<double>& eig) {
+  const int nt = (int)tchain.size();
+  eig.assign(nt, 0.0);
+  if (nt == 0) return RP_OK;
+  int rc;
+  DevBuf<double> d_mid, d_ldt;
+  DevBuf<int> d_map, d_cnt;
+  auto cleanup = [&]() { d_mid.free(); d_ldt.free(); d_map.free(); d_cnt.free(); };
+  if ((rc = d_mid.alloc(nt)) || (rc = d_ldt.alloc(nt)) || (rc = d_map.alloc(nt)) ||
+      (rc = d_cnt.alloc(nt)) || (rc = d_map.upload(tchain.data(), nt, r->s))) {
+    cleanup();
+    return rc;
+  }
+  std::vector<double> mid(nt);
+  std::vector<int> k(nt);
+  // 200 halvings take any bracket of doubles to adjacent values; the loop ends sooner, when
+  // every bracket is within a few ulps
+  for (int it = 0; !rc && it < 200; ++it) {
+    bool open = false;
+    for (int t = 0; t < nt; ++t) {
+      mid[t] = 0.5 * (a[t] + b[t]);
+      open |= (b[t] - a[t]) > 4e-16 * std::max(std::fabs(a[t]), std::fabs(b[t]));
+    }
+    if (!open) break;
+    if ((rc = d_mid.upload(mid.data(), nt, r->s))) break;
+    rc = with_logdet_kernel(r->D, [&](auto kern, int nth) -> int {
+      kern<<<nt, nth, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p, r->d_massd.p,
+                                 r->d_blocks.p, d_mid.p, nullptr, d_ldt.p, d_cnt.p, d_map.p);
+      RP_HIP(hipGetLastError());
+      return RP_OK;
+    });
+    if (rc || (rc = d_cnt.download(k.data(), nt, r->s))) break;
+    if (hipStreamSynchronize(r->s) != hipSuccess) {
+      rc = set_error(RP_ERR_HIP, "inertia count");
+      break;
+    }
+    for (int t = 0; t < nt; ++t) (k[t] > tj[t] ? b[t] : a[t]) = mid[t];
+  }
+  for (int t = 0; t < nt; ++t) eig[t] = 0.5 * (a[t] + b[t]);
+  cleanup();
+  return rc;
+}
+
+// The negative eigenvalues of the chains listed (cnt[c] of them for chain c), each bracketed
+// between the chain's Gershgorin lower bound and 0.  sumlog[c] gets sum ln|eta| over them,
+// lowest[c] the smallest one.
+int ring_negative_eigs(rp_ring* r, const std::vector<int>& chains, const std::vector<int>& cnt,
+                       std::vector<double>& sumlog, std::vector<double>& lowest) {
+  if (chains.empty()) return RP_OK;
+  int rc;
+  std::vector<double> lohi;
+  if ((rc = ring_gersh(r, lohi))) return rc;
+  std::vector<int> tchain, tj;
+  std::vector<double> a, b, eig;
+  for (int c : chains)
+    for (int j = 0; j < cnt[c]; ++j) {
+      tchain.push_back(c);
+      tj.push_back(j);
+      a.push_back(lohi[2 * c] - 1e-12 * std::fabs(lohi[2 * c]) - 1e-300);
+      b.push_back(0.0);
+    }
+  if ((rc = ring_bisect(r, tchain, tj, a, b, eig))) return rc;
+  for (size_t t = 0; t < tchain.size(); ++t) {
+    const int c = tchain[t];
+    if (!(eig[t] < 0.0))
+      return set_error(RP_ERR_CONVERGE, "chain %d: the inertia bisection lost eigenvalue %d", c, tj[t]);
+    sumlog[c] += std::log(std::fabs(eig[t]));
+    lowest[c] = std::min(lowest[c], eig[t]);
+  }
+  return RP_OK;
+}
+}  // namespace rp
+
+extern "C" int rp_ring_lowest_eigs(rp_ring* r, int singlewell, int k, double* eigs) {
+  RP_REQUIRE(r && eigs && k >= 1, "rp_ring_lowest_eigs: bad arguments");
+  for (int c = 0; c < r->nchain; ++c)
+    RP_REQUIRE(k <= r->n[c] * r->D, "rp_ring_lowest_eigs: chain %d has only %d eigenvalues", c,
+               r->n[c] * r->D);
+  int rc;
+  if ((rc = ring_blocks(r, singlewell))) return rc;
+  std::vector<double> lohi;
+  if ((rc = ring_gersh(r, lohi))) return rc;
+  std::vector<int> tchain, tj;
+  std::vector<double> a, b, eig;
+  for (int c = 0; c < r->nchain; ++c)
+    for (int j = 0; j < k; ++j) {
+      tchain.push_back(c);
+      tj.push_back(j);
+      const double lo = lohi[2 * c], hi = lohi[2 * c + 1];
+      a.push_back(lo - 1e-12 * std::fabs(lo) - 1e-300);
+      b.push_back(hi + 1e-12 * std::fabs(hi) + 1e-300);
+    }
+  if ((rc = ring_bisect(r, tchain, tj, a, b, eig))) return rc;
+  for (size_t t = 0; t < eig.size(); ++t) eigs[t] = eig[t];
+  return RP_OK;
+}
+
 extern "C" int rp_ring_logdet(rp_ring* r, int singlewell, int skip_lowest, double* logdet_pos,
                               int* nneg, double* eta1) {
   RP_REQUIRE(r && logdet_pos, "rp_ring_logdet: bad arguments");
@@ -874,7 +1178,8 @@ extern "C" int rp_ring_logdet(rp_ring* r, int singlewell, int skip_lowest, doubl
     return rc;
   if ((rc = with_logdet_kernel(D, [&](auto kern, int nt) -> int {
         kern<<<r->nchain, nt, 0, r->s>>>(D, r->d_n.p, r->d_boff.p, r->d_betan.p, r->d_massd.p,
-                                        r->d_blocks.p, nullptr, r->d_sinv.p, r->d_ld.p, r->d_nneg.p);
+                                        r->d_blocks.p, nullptr, r->d_sinv.p, r->d_ld.p, r->d_nneg.p,
+                                        nullptr);
         RP_HIP(hipGetLastError());
         return RP_OK;
       })))
@@ -885,30 +1190,35 @@ extern "C" int rp_ring_logdet(rp_ring* r, int singlewell, int skip_lowest, doubl
       (rc = r->d_nneg.download(ng.data(), r->nchain, r->s)))
     return rc;
   RP_HIP(hipStreamSynchronize(r->s));
-  if (skip_lowest || eta1) {
+  const bool want_near = skip_lowest || eta1;
+  if (want_near) {
     if ((rc = ring_lowest_eigen(r, et.data()))) return rc;
   }
-  int status = RP_OK;
+  // log|det| = sum ln|eta| over every eigenvalue, so the sum over the positive ones is log|det|
+  // less ln|eta| over the negative ones.  With one negative eigenvalue that is the eigenvalue
+  // nearest zero when inverse iteration found a negative one; every other negative spectrum
+  // is bisected on the inertia.
+  std::vector<int> bis;
+  std::vector<double> neglog(r->nchain, 0.0), lowest(r->nchain, INFINITY);
   for (int c = 0; c < r->nchain; ++c) {
-    double v = ld[c];
-    if (skip_lowest) {
-      // sum over eta_i > 0, i >= 2 = log|det| - log|eta_1| - sum over the other eta < 0
-      if (ng[c] == 0 || (ng[c] == 1 && et[c] < 0.0)) {
-        v = ld[c] - std::log(std::fabs(et[c]));
-      } else {
-        status = set_error(RP_ERR_CONVERGE,
-                           "chain %d: %d negative eigenvalues; their sum is not fixed by det", c, ng[c]);
-        v = NAN;
-      }
-    } else if (ng[c] != 0) {
-      status = set_error(RP_ERR_CONVERGE, "chain %d: %d negative eigenvalues", c, ng[c]);
-      v = NAN;
+    if (ng[c] == 1 && want_near && et[c] < 0.0) {
+      neglog[c] = std::log(-et[c]);
+      lowest[c] = et[c];
+    } else if (ng[c] >= 1) {
+      bis.push_back(c);
     }
+  }
+  if ((rc = ring_negative_eigs(r, bis, ng, neglog, lowest))) return rc;
+  for (int c = 0; c < r->nchain; ++c) {
+    double v = ld[c] - neglog[c];  // sum of ln(eta) over eta > 0 (rpi_ser.f90:227-236)
+    // lndetJ (rpi_ser.f90:355-365) also skips eta_1, the lowest eigenvalue: with every
+    // eigenvalue positive that is the one nearest zero
+    if (skip_lowest && ng[c] == 0) v -= std::log(std::fabs(et[c]));
     logdet_pos[c] = v;
     if (nneg) nneg[c] = ng[c];
-    if (eta1) eta1[c] = et[c];
+    if (eta1) eta1[c] = ng[c] == 0 ? et[c] : lowest[c];
   }
-  return status;
+  return RP_OK;
 }
 
 extern "C" int rp_ring_param_grad(rp_ring* r, double* out, int* nparam) {

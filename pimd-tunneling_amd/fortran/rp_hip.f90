@@ -11,7 +11,7 @@ module rp_hip
   integer(c_int), parameter :: RP_OK = 0, RP_ERR_ARG = 1, RP_ERR_HIP = 2, RP_ERR_NAN = 3, &
        RP_ERR_NOISE = 4, RP_ERR_STATE = 5, RP_ERR_CONVERGE = 6
   integer(c_int), parameter :: RP_PES_QUARTIC = 1, RP_PES_COUPLED2D = 2, RP_PES_EVB = 3, &
-       RP_PES_WATER = 4, RP_PES_HARMONIC = 5
+       RP_PES_WATER = 4, RP_PES_HARMONIC = 5, RP_PES_HOST = 6
   integer(c_int), parameter :: RP_THERMO_ANDERSEN = 1, RP_THERMO_LANGEVIN = 2
 
   type, bind(C) :: rp_polymer_desc
@@ -51,6 +51,16 @@ module rp_hip
        integer(c_int), intent(in) :: ip(*)
        type(c_ptr), intent(out) :: out
      end function rp_pes_create
+     ! a plugin that lives on the host: potforce / hessian are bind(C) callbacks
+     ! (see rp_bridge_host.f90), user is passed back to them unchanged
+     integer(c_int) function rp_pes_create_host(ndim, natom, potforce, hessian, user, out) &
+          bind(C, name="rp_pes_create_host")
+       import :: c_int, c_ptr, c_funptr
+       integer(c_int), value :: ndim, natom
+       type(c_funptr), value :: potforce, hessian
+       type(c_ptr), value :: user
+       type(c_ptr), intent(out) :: out
+     end function rp_pes_create_host
      integer(c_int) function rp_pes_destroy(pes) bind(C, name="rp_pes_destroy")
        import :: c_int, c_ptr
        type(c_ptr), value :: pes
@@ -130,6 +140,18 @@ module rp_hip
        type(c_ptr), value :: c
        integer(c_int64_t), intent(out) :: cursor
      end function rp_chain_get_cursor
+     integer(c_int) function rp_chain_set_kick_cursor(c, kick_cursor) &
+          bind(C, name="rp_chain_set_kick_cursor")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: c
+       integer(c_int64_t), value :: kick_cursor
+     end function rp_chain_set_kick_cursor
+     integer(c_int) function rp_chain_get_kick_cursor(c, kick_cursor) &
+          bind(C, name="rp_chain_get_kick_cursor")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: c
+       integer(c_int64_t), intent(out) :: kick_cursor
+     end function rp_chain_get_kick_cursor
      integer(c_int) function rp_nm_transform(c, direction, ncols, shift_dof, in, out) &
           bind(C, name="rp_nm_transform")
        import :: c_int, c_ptr, c_double
@@ -351,6 +373,13 @@ module rp_hip
        real(c_double), intent(out) :: logdet_pos(*), eta1(*)
        integer(c_int), intent(out) :: nneg(*)
      end function rp_ring_logdet
+     integer(c_int) function rp_ring_lowest_eigs(r, singlewell, k, eigs) &
+          bind(C, name="rp_ring_lowest_eigs")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: r
+       integer(c_int), value :: singlewell, k
+       real(c_double), intent(out) :: eigs(*)
+     end function rp_ring_lowest_eigs
      integer(c_int) function rp_instanton(r, pgtol, factr, m, maxiter, x_inout, action, iters, &
           status) bind(C, name="rp_instanton")
        import :: c_int, c_ptr, c_double

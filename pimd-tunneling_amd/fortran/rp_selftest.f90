@@ -7,6 +7,7 @@
 !         a(ndof) / b(ndof) / dbdl(ndof) / x(n*ndof) / p(n*ndof)   (x, p as x(n,ndim,natom))
 ! stdout: PLUGIN  V, Vprime, Vdoubleprime at bead 1
 !         PILE    x, p, dHdr after propagate_pimd_pile from (x, p), Philox seed, stream 0
+!         PILE2   the same after a second init_nm, from the same (x, p), continuing the stream
 !         UM      UMforceenergy energy and gradient at x with ends a, b
 !         BAND    UMhessian(x, singlewell=.false.)
 !         GAULEG  11 nodes and weights on [0, 1]
@@ -19,7 +20,7 @@ program rp_selftest
   integer(8):: seed
   double precision:: v0in, e, dHdr, xg(11), wg(11)
   double precision, allocatable:: dp(:), a(:,:), b(:,:), dbdl(:,:), x(:,:,:), p(:,:,:)
-  double precision, allocatable:: x0(:,:,:), g(:,:,:), band(:,:), grad1(:,:), hess1(:,:,:,:)
+  double precision, allocatable:: x0(:,:,:), p0(:,:,:), g(:,:,:), band(:,:), grad1(:,:), hess1(:,:,:,:)
   integer, allocatable:: ip(:)
 
   read(*,*) kind, ndim, natom, ndp, nip
@@ -59,8 +60,20 @@ program rp_selftest
   call init_nm(a, b)
   allocate(x0(n,ndim,natom))
   x0 = x
+  allocate(p0(n,ndim,natom))
+  p0 = p
   call propagate_pimd_pile(x, p, a, b, dbdl, 0.0d0, dHdr)
   write(*,'(A)') 'PILE'
+  write(*,'(ES25.17)') x
+  write(*,'(ES25.17)') p
+  write(*,'(ES25.17)') dHdr
+  ! the next task of the process (pimd_par.f90:321-329): init_nm again, from the same
+  ! start; its draws continue the stream, so it is not a replay of the first task
+  call init_nm(a, b)
+  x = x0
+  p = p0
+  call propagate_pimd_pile(x, p, a, b, dbdl, 0.0d0, dHdr)
+  write(*,'(A)') 'PILE2'
   write(*,'(ES25.17)') x
   write(*,'(ES25.17)') p
   write(*,'(ES25.17)') dHdr

@@ -1,33 +1,39 @@
 ! verletint_rp.f90 -- `module verletint` (verletmodule.f90) on the GPU.
 !
 ! The reference drivers set this module's variables (NMC, Noutput, imin, dt, dHdrlimit,
-! tau, gamma, cayley, ...) and call alloc_nm, init_nm, init_path, propagate_pimd_pile /
-! propagate_pimd_nm and free_nm; the same names do the same here, each one a launch
-! sequence on the device chain of libpimdtunnel.so.  The state arrays keep the
-! reference layout x(n, ndim, natom) and are copied to and from the device per call.
+! tau, gamma, cayley, restart, ...) and call alloc_nm, init_nm, init_path,
+! propagate_pimd_pile / propagate_pimd_nm, write_restart and free_nm; the same names do the
+! same here, each one a launch sequence on the device chain of libpimdtunnel.so.  The state
+! arrays keep the reference layout x(n, ndim, natom) and are copied to and from the device
+! per call.  The potential is reached through `module rp_bridge` (a device plugin, or any
+! host plugin of the reference's API).
 !
 ! Random numbers: the reference seeds MKL VSL MT19937 streams from the clock
-! (verletmodule.f90:355-358).  Here every trajectory draws from a counter-based Philox
-! stream (seed rp_seed, stream rp_stream); alloc_nm takes the seed from the clock as the
-! reference does unless the driver has set rp_seed >= 0.
+! (verletmodule.f90:355-358) and draws from them for the whole run.  Here every process
+! draws from one counter-based Philox stream (seed rp_seed, stream rp_stream = iproc):
+! alloc_nm takes the seed from the clock as the reference does unless the driver has set
+! rp_seed >= 0, and the stream positions (rp_cursor for the normals, rp_kick_cursor for the
+! Andersen intervals) carry over from one task's init_nm to the next, so successive tasks of
+! a process draw fresh numbers as the VSL streams do.
 !
-! Not carried over: write_restart / restart (the state can be read back with
-! rp_chain_get_state), the NR-FFT transform selected by use_mkl (the transform is the
-! dense DST-I on the device either way) and the readhess initial momenta.
+! Not carried over: the NR-FFT transform selected by use_mkl (the transform is the dense
+! DST-I on the device either way) and the readhess initial momenta.
 module verletint
   use iso_c_binding
   use rp_hip
   use mcmod_mass
+  use rp_bridge
   use instantonmod
   implicit none
 
   integer::                         NMC, Noutput, imin, restart=0, iproc=0, restartnmc=0, restartunit
   double precision::                dt, dHdrlimit=-1.0d0
   double precision, allocatable::   transmatrix(:,:), beadvec(:,:), beadmass(:,:), lam(:)
-  double precision, allocatable::   path(:,:,:), lampath(:)
+  double precision, allocatable::   path(:,:,:), lampath(:), splinepath(:,:,:), Vpath(:)
+  double precision, allocatable::   hesspath(:,:,:), splinehess(:,:,:)
   double precision::                tau=1.0d0, gamma=0.0d0
   logical::                         iprint=.false., use_mkl=.false., cayley=.false., readhess=.false.
-  integer(c_int64_t)::              rp_seed=-1, rp_stream=0
+  integer(c_int64_t)::              rp_seed=-1, rp_stream=0, rp_cursor=0, rp_kick_cursor=0
   type(c_ptr)::                     rp_chain_handle = c_null_ptr
 
 contains
@@ -41,18 +47,31 @@ contains
        rp_seed = int(mod(itime + 5*iproc, 1000), c_int64_t)
     end if
     rp_stream = int(iproc, c_int64_t)
+    rp_cursor = 0
+    rp_kick_cursor = 0
     write(*,*) "proc", iproc, "running with Philox seed", rp_seed, "stream", rp_stream
   end subroutine alloc_nm
+
+  ! the device chain's stream position is kept when the chain goes away
+  subroutine drop_chain(where)
+    character(len=*), intent(in):: where
+    if (.not. c_associated(rp_chain_handle)) return
+    call rp_check(rp_chain_get_cursor(rp_chain_handle, rp_cursor), where)
+    call rp_check(rp_chain_get_kick_cursor(rp_chain_handle, rp_kick_cursor), where)
+    call rp_check(rp_chain_destroy(rp_chain_handle), where)
+    rp_chain_handle = c_null_ptr
+  end subroutine drop_chain
 
   ! init_nm (verletmodule.f90:306-338) plus the Langevin table of :380-386
   subroutine init_nm(a, b)
     double precision, intent(in):: a(:,:), b(:,:)
     double precision, allocatable, target:: T(:,:), lm(:), bm(:,:), bv(:,:)
-    if (c_associated(rp_chain_handle)) call rp_check(rp_chain_destroy(rp_chain_handle), 'init_nm')
-    call rp_check(rp_nm_setup(rp_pes_handle, int(n, c_int), int(ndim, c_int), int(natom, c_int), &
+    call drop_chain('init_nm')
+    call rp_check(rp_nm_setup(rp_bridge_pes(), int(n, c_int), int(ndim, c_int), int(natom, c_int), &
          mass, betan, tau, dt, gamma, merge(1_c_int, 0_c_int, cayley), a, b, rp_chain_handle), &
          'init_nm')
-    call rp_check(rp_chain_set_rng(rp_chain_handle, rp_seed, rp_stream, 0_c_int64_t), 'init_nm')
+    call rp_check(rp_chain_set_rng(rp_chain_handle, rp_seed, rp_stream, rp_cursor), 'init_nm')
+    call rp_check(rp_chain_set_kick_cursor(rp_chain_handle, rp_kick_cursor), 'init_nm')
     allocate(T(n, n), lm(n), bm(natom, n), bv(n, ndof))
     call rp_check(rp_chain_get_nm(rp_chain_handle, c_loc(T), c_loc(lm), c_loc(bm), c_loc(bv)), &
          'init_nm')
@@ -66,8 +85,7 @@ contains
   end subroutine init_nm
 
   subroutine free_nm()
-    if (c_associated(rp_chain_handle)) call rp_check(rp_chain_destroy(rp_chain_handle), 'free_nm')
-    rp_chain_handle = c_null_ptr
+    call drop_chain('free_nm')
     if (allocated(transmatrix)) deallocate(transmatrix, beadvec, beadmass, lam)
   end subroutine free_nm
 
@@ -92,15 +110,74 @@ contains
     call get_state(x, p)
   end subroutine init_path
 
-  ! propagate_pimd_pile (verletmodule.f90:372-416)
+  ! write_restart (verletmodule.f90:162-185): positions with the running TI sum, then
+  ! momenta with the step count, one block of natom lines per bead, on restartunit
+  subroutine write_restart(xprop, pprop, ii, dHdr)
+    double precision, intent(in):: xprop(:,:,:), pprop(:,:,:), dHdr
+    integer, intent(in):: ii
+    integer:: i, j
+    rewind(restartunit)
+    do i = 1, n
+       write(restartunit,*) natom
+       write(restartunit,*) dHdr
+       do j = 1, natom
+          write(restartunit,*) label(j), xprop(i,1,j), xprop(i,2,j), xprop(i,3,j)
+       end do
+    end do
+    do i = 1, n
+       write(restartunit,*) natom
+       write(restartunit,*) ii
+       do j = 1, natom
+          write(restartunit,*) label(j), pprop(i,1,j), pprop(i,2,j), pprop(i,3,j)
+       end do
+    end do
+  end subroutine write_restart
+
+  ! propagate_pimd_pile (verletmodule.f90:372-416).  dHdr is the running TI sum on entry
+  ! when resuming (restart = 2, pimd_par.f90:355-370) and the mean on return.  With
+  ! restart > 0 the trajectory runs in device segments between the reference's output
+  ! points (every Noutput steps after imin, :389-395), where write_restart records the
+  ! state before that step; the segments continue one stream, so the trajectory is the
+  ! same as one uninterrupted call.
   subroutine propagate_pimd_pile(xprop, vprop, a, b, dbdl, xi, dHdr)
     double precision:: xprop(:,:,:), vprop(:,:,:), a(:,:), b(:,:), dbdl(:,:), xi, dHdr
-    integer(c_int64_t):: nre
+    integer(c_int64_t):: nre, nretot
+    integer:: ii, done, cnt
+    double precision:: seg, total
+    total = 0.0d0
+    if (restart >= 2) total = dHdr
+    nretot = 0
+    done = 0
+    cnt = 0
     call put_state(xprop, vprop)
-    call rp_check(rp_propagate_pile(rp_chain_handle, int(NMC, c_int64_t), int(imin, c_int64_t), &
-         dHdrlimit, dbdl, xi, c_null_ptr, 0_c_int64_t, dHdr, nre), 'propagate_pimd_pile')
-    if (nre > 0) write(*,*) "Over limit: path re-initialised", nre, "times"
+    if (restart > 0 .and. Noutput > 0) then
+       do ii = 1, NMC
+          cnt = cnt + 1
+          if (cnt >= Noutput .and. ii > imin) then
+             cnt = 0
+             call run_segment(ii - 1 - done)
+             call get_state(xprop, vprop)
+             call write_restart(xprop, vprop, ii + restartnmc, total)
+          end if
+       end do
+    end if
+    call run_segment(NMC - done)
+    if (nretot > 0) write(*,*) "Over limit: path re-initialised", nretot, "times"
     call get_state(xprop, vprop)
+    dHdr = total/dble(NMC + restartnmc - imin)
+  contains
+    ! steps done+1 .. done+nseg of the trajectory; TI after global step imin
+    subroutine run_segment(nseg)
+      integer, intent(in):: nseg
+      integer:: imloc
+      if (nseg <= 0) return
+      imloc = min(max(imin - done, 0), nseg)
+      call rp_check(rp_propagate_pile(rp_chain_handle, int(nseg, c_int64_t), int(imloc, c_int64_t), &
+           dHdrlimit, dbdl, xi, c_null_ptr, 0_c_int64_t, seg, nre), 'propagate_pimd_pile')
+      if (nseg > imloc) total = total + seg*dble(nseg - imloc)
+      nretot = nretot + nre
+      done = done + nseg
+    end subroutine run_segment
   end subroutine propagate_pimd_pile
 
   ! propagate_pimd_nm (verletmodule.f90:190-250), the Andersen thermostat

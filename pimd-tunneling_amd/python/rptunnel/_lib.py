@@ -35,12 +35,17 @@ class PolymerDesc(C.Structure):
                 ("b", dp), ("dbdl", dp), ("stream", C.c_uint64)]
 
 
+# Host-potential callbacks (rp_host_potforce_fn / rp_host_hessian_fn of rp_hip.h)
+HOST_POTFORCE = C.CFUNCTYPE(C.c_int, vp, C.c_int, dp, dp, dp)
+HOST_HESSIAN = C.CFUNCTYPE(C.c_int, vp, C.c_int, dp, dp)
+
 _SIGS = {
     "rp_init": [C.c_int],
     "rp_finalize": [],
     "rp_last_error": [C.c_char_p, C.c_int],
     "rp_version": [],
     "rp_pes_create": [C.c_int, C.c_int, C.c_int, dp, C.c_int, ip, C.c_int, C.POINTER(vp)],
+    "rp_pes_create_host": [C.c_int, C.c_int, HOST_POTFORCE, HOST_HESSIAN, vp, C.POINTER(vp)],
     "rp_pes_destroy": [vp],
     "rp_pes_set_v0": [vp, C.c_double],
     "rp_pes_potforce": [vp, C.c_int, dp, dp, dp],
@@ -55,6 +60,8 @@ _SIGS = {
     "rp_chain_get_nm": [vp, dp, dp, dp, dp],
     "rp_chain_set_rng": [vp, C.c_uint64, C.c_uint64, C.c_int64],
     "rp_chain_get_cursor": [vp, i64p],
+    "rp_chain_set_kick_cursor": [vp, C.c_int64],
+    "rp_chain_get_kick_cursor": [vp, i64p],
     "rp_nm_transform": [vp, C.c_int, C.c_int, ip, dp, dp],
     "rp_step_v": [vp, C.c_double],
     "rp_step_nm": [vp, C.c_double],
@@ -96,6 +103,7 @@ _SIGS = {
     "rp_ring_hvp": [vp, dp, dp],
     "rp_ring_hessian_band": [vp, C.c_int, C.c_int, dp],
     "rp_ring_logdet": [vp, C.c_int, C.c_int, dp, ip, dp],
+    "rp_ring_lowest_eigs": [vp, C.c_int, C.c_int, dp],
     "rp_instanton": [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, ip, ip],
     "rp_ring_param_grad": [vp, dp, ip],
 }
@@ -154,6 +162,8 @@ def f64(a):
 def ptr(a):
     if a is None:
         return None
+    if not a.flags.c_contiguous:  # the library reads plain memory: a strided view would alias
+        raise ValueError("ptr: array is not contiguous")
     return a.ctypes.data_as(dp)
 
 

@@ -99,8 +99,9 @@ class Ring:
 
     def logdet(self, singlewell=False, skip_lowest=True, want_eta=None):
         """-> (sum of ln(eta) over the eigenvalues the reference keeps, nneg, eta_1).
-        eta_1 (the lowest eigenvalue, by inverse iteration) is computed when skip_lowest
-        needs it or want_eta asks for it; otherwise it is returned as NaN."""
+        eta_1 (the lowest eigenvalue: inverse iteration finds the one nearest zero, bisection
+        on the inertia any other negative ones) is computed when skip_lowest needs it or
+        want_eta asks for it; otherwise it is returned as NaN."""
         if want_eta is None:
             want_eta = skip_lowest
         ld = np.zeros(self.nchain)
@@ -109,6 +110,13 @@ class Ring:
         L.check(L.lib().rp_ring_logdet(self.h, int(singlewell), int(skip_lowest), L.ptr(ld),
                                        L.iptr(nn), L.ptr(eta) if want_eta else None))
         return ld, nn, eta
+
+    def lowest_eigs(self, k, singlewell=False):
+        """The k lowest eigenvalues of every chain's band (detJ's ascending spectrum,
+        instantonmod.f90:782-827, head only): shape (nchain, k)."""
+        out = np.zeros(self.nchain * k)
+        L.check(L.lib().rp_ring_lowest_eigs(self.h, int(singlewell), int(k), L.ptr(out)))
+        return out.reshape(self.nchain, k)
 
     def instanton(self, chains, pgtol=1e-5, factr=1e6, m=8, maxiter=0):
         """L-BFGS on every chain at once -> (optimised chains, U_M, iterations, status)."""

@@ -15,6 +15,9 @@ from .specs import PesSpec
 
 
 class Pes:
+    """A device potential (``spec``), or with ``Pes.host`` a potential that lives on the
+    host: the caller's own plugin, called back by the library for whole chains."""
+
     def __init__(self, spec: PesSpec):
         L.init()
         self.spec = spec
@@ -26,6 +29,46 @@ class Pes:
                                       L.iptr(ipa), 0 if ipa is None else ipa.size, C.byref(h)))
         self.h = h
         self.V0 = 0.0
+
+    @classmethod
+    def host(cls, ndim, natom, potforce, hessian=None):
+        """A plugin with no device twin (mcmod_waterdimer.f90:12-103's role, in Python):
+        potforce(x) with x of shape (ndof, nbead) returns (grad (ndof, nbead), energy
+        (nbead,)) with the plugin's own V - V0; hessian(x) (optional) returns
+        (nbead, ndof, ndof) with [b, i, k] = d grad_k / d x_i.  Without it the library
+        differences potforce (step 1e-4).  The plain path and the ring routines run with
+        it; the batched engine refuses it."""
+        self = cls.__new__(cls)
+        L.init()
+        self.spec = None
+        self.ndim, self.natom, self.ndof = ndim, natom, ndim * natom
+        D = self.ndof
+
+        def _pf(user, nbead, x, grad, energy):
+            try:
+                xa = np.ctypeslib.as_array(x, shape=(D * nbead,)).reshape(D, nbead)
+                g, e = potforce(xa.copy())
+                np.ctypeslib.as_array(grad, shape=(D * nbead,))[:] = np.asarray(g, float).reshape(-1)
+                np.ctypeslib.as_array(energy, shape=(nbead,))[:] = np.asarray(e, float).reshape(-1)
+                return 0
+            except Exception:
+                return 1
+
+        def _hs(user, nbead, x, hess):
+            try:
+                xa = np.ctypeslib.as_array(x, shape=(D * nbead,)).reshape(D, nbead)
+                h = np.asarray(hessian(xa.copy()), float).reshape(-1)
+                np.ctypeslib.as_array(hess, shape=(nbead * D * D,))[:] = h
+                return 0
+            except Exception:
+                return 1
+
+        self._cb = (L.HOST_POTFORCE(_pf), L.HOST_HESSIAN(_hs) if hessian else L.HOST_HESSIAN())
+        h = C.c_void_p()
+        L.check(L.lib().rp_pes_create_host(ndim, natom, self._cb[0], self._cb[1], None, C.byref(h)))
+        self.h = h
+        self.V0 = 0.0
+        return self
 
     def close(self):
         if getattr(self, "h", None):
@@ -49,8 +92,8 @@ class Pes:
 
     def _beads(self, x):
         x = np.asarray(x, dtype=np.float64)
-        if x.ndim == 1:
-            return x.reshape(self.ndof, 1), 1
+        if x.ndim == 1:  # a column of a chain is a strided view: copy it to contiguous memory
+            return np.ascontiguousarray(x).reshape(self.ndof, 1), 1
         return np.ascontiguousarray(x), x.shape[1]
 
     def potforce_beads(self, x):

@@ -38,7 +38,7 @@ def _sections(text):
     out, key = {}, None
     for line in text.splitlines():
         s = line.strip()
-        if s in ("PLUGIN", "PILE", "UM", "BAND", "GAULEG", "END"):
+        if s in ("PLUGIN", "PILE", "PILE2", "UM", "BAND", "GAULEG", "END"):
             key = s
             out[key] = []
         elif key and s:
@@ -71,11 +71,19 @@ def test_fortran_dropin_against_oracle(name):
     assert rel(pl[1 + D:].reshape(D, D), op.Vdoubleprime(x[:, 0])) <= 1e-11
     # propagate_pimd_pile from (x, p): Philox seed, stream 0 (alloc_nm(0))
     nm = O.NM(n, sp.ndim, sp.natom, sp.mass, betan, tau, dt=dt, gamma=gamma, a=sp.well1, b=sp.well2)
-    xo, po, dho, _ = O.propagate_pile(nm, op, O.Rng(seed=seed, stream=0), x, p, nmc, dbdl, imin=imin)
+    rng = O.Rng(seed=seed, stream=0)
+    xo, po, dho, _ = O.propagate_pile(nm, op, rng, x, p, nmc, dbdl, imin=imin)
     pile = r["PILE"]
     assert rel(pile[:D * n].reshape(D, n), xo) <= 1e-10
     assert rel(pile[D * n:2 * D * n].reshape(D, n), po) <= 1e-10
     assert abs(pile[-1] - dho) <= 1e-10 * abs(dho)
+    # the second task continues the same stream: different draws, the oracle's next ones
+    xo2, po2, dho2, _ = O.propagate_pile(nm, op, rng, x, p, nmc, dbdl, imin=imin)
+    pile2 = r["PILE2"]
+    assert rel(pile2[:D * n].reshape(D, n), xo2) <= 1e-10
+    assert rel(pile2[D * n:2 * D * n].reshape(D, n), po2) <= 1e-10
+    assert abs(pile2[-1] - dho2) <= 1e-10 * abs(dho2)
+    assert rel(pile2[:D * n].reshape(D, n), xo) > 1e-6  # not a replay of the first task
     oc = O.Chain(n, sp.ndim, sp.natom, sp.mass, betan, True)
     go, eo = oc.UMforceenergy(op, x, sp.well1, sp.well2)
     assert abs(r["UM"][0] - eo) <= 1e-12 * abs(eo)

@@ -9,7 +9,7 @@ same comparisons also run live on fresh random chains.
 Bars: action and gradients 1e-13 relative (same formula, same FP64 order up to
 compiler contraction); bands 1e-12 where both sides use analytic Hessians and 1e-9 for
 the EVB (the reference plugin differentiates Vprime by central differences, eps = 1e-4,
-mcmod_waterdimer.f90:143-163; the oracle's EVB Hessian is analytic).
+mcmod_waterdimer.f90:83-103; the oracle's EVB Hessian is analytic).
 """
 import os
 
