@@ -72,6 +72,15 @@ int rp_pes_set_v0(rp_pes *pes, double v0);
  * x and grad are (nbead, ndim, natom) column-major; energy[nbead] (V - V0); grad may be
  * NULL (V only), energy may be NULL (Vprime only). */
 int rp_pes_potforce(rp_pes *pes, int nbead, const double *x, double *grad, double *energy);
+/* rp_pes_potforce without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid it).  Same arguments and results as rp_pes_potforce for a
+ * device potential; the staging of x, grad and energy lives in one static device array of
+ * 2^20 doubles per GPU, so nbead * (2 * ndof + 1) must stay within 2^20 (RP_ERR_ARG
+ * otherwise; 19,065 beads at ndof = 27).  Calls that overlap on one GPU share that array
+ * and race.  A potential that lives on the host gets RP_ERR_ARG. */
+int rp_pes_potforce_noalloc(rp_pes *pes, int nbead, const double *x, double *grad,
+                            double *energy);
 /* Vdoubleprime (mcmod_waterdimer.f90:83-103) for nbead beads: hess[bead][ndof][ndof] with
  * hess[b][i][k] = d grad_k / d x_i, from forward-over-reverse (exact, no finite steps). */
 int rp_pes_hessian(rp_pes *pes, int nbead, const double *x, double *hess);

@@ -12,6 +12,16 @@ static thread_local std::string g_err;
 static int g_device = -1;
 static std::mutex g_init_mu;
 
+// Staging memory of rp_pes_potforce_noalloc, the entry point that allocates nothing: one
+// array per device (each device holds its own copy until the process ends), 2^20 doubles =
+// 8 MiB.  Host code never passes the array's name to a kernel or a copy (on the host it is
+// only a stand-in): rp_init takes each device's address of it once, with that device
+// current, and every use goes through g_scratch_ptr.
+constexpr size_t kScratchDoubles = size_t(1) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static double* g_scratch_ptr[kMaxDevices] = {};
+
 int set_error(int code, const char* fmt, ...) {
   char buf[1024];
   va_list ap;
@@ -264,6 +274,11 @@ extern "C" int rp_init(int device) {
   if (device < 0 || device >= count)
     return set_error(RP_ERR_ARG, "device %d out of range (count %d)", device, count);
   RP_HIP(hipSetDevice(device));
+  if (device < kMaxDevices && !g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    RP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
   g_device = device;
   return RP_OK;
 }
@@ -445,6 +460,40 @@ extern "C" int rp_pes_potforce(rp_pes* pes, int nbead, const double* x, double* 
   if (grad && (rc = pes->d_g.download(grad, nx, pes->stream))) return rc;
   if (energy && (rc = pes->d_e.download(energy, nbead, pes->stream))) return rc;
   RP_HIP(hipStreamSynchronize(pes->stream));
+  return RP_OK;
+}
+
+// rp_pes_potforce with its staging in the static scratch array instead of the potential's
+// own device buffers: no allocation, so callers that cannot allocate (graph capture, tight
+// serving loops) can call it.  x, grad and energy are host arrays as for rp_pes_potforce.
+// Every call on a device shares that device's array: calls that overlap on one device (two
+// threads, or two potentials' streams) race, and nothing of a call survives the next.
+extern "C" int rp_pes_potforce_noalloc(rp_pes* pes, int nbead, const double* x, double* grad,
+                                       double* energy) {
+  RP_REQUIRE(pes && x && nbead >= 0, "rp_pes_potforce_noalloc: bad arguments");
+  RP_REQUIRE(pes->kind != PES_HOST,
+             "rp_pes_potforce_noalloc: a potential that lives on the host has nothing to stage "
+             "on the device (call rp_pes_potforce)");
+  if (nbead == 0) return RP_OK;
+  const size_t nx = (size_t)nbead * pes->ndof;
+  const size_t need = 2 * nx + (size_t)nbead;
+  RP_REQUIRE(need <= kScratchDoubles,
+             "rp_pes_potforce_noalloc: %d beads of %d dof need %zu doubles of staging; the "
+             "limit is %zu (at most %zu beads)", nbead, pes->ndof, need, kScratchDoubles,
+             kScratchDoubles / (2 * (size_t)pes->ndof + 1));
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  RP_REQUIRE(g_device >= 0 && g_device < kMaxDevices && g_scratch_ptr[g_device],
+             "rp_pes_potforce_noalloc: no staging array on device %d", g_device);
+  double* dx = g_scratch_ptr[g_device];
+  double* dg = dx + nx;
+  double* de = dg + nx;
+  hipStream_t s = pes->stream;
+  RP_HIP(hipMemcpyAsync(dx, x, nx * sizeof(double), hipMemcpyHostToDevice, s));
+  if ((rc = launch_pes_eval(pes, nbead, nbead, dx, grad ? dg : nullptr, de, s))) return rc;
+  if (grad) RP_HIP(hipMemcpyAsync(grad, dg, nx * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (energy) RP_HIP(hipMemcpyAsync(energy, de, nbead * sizeof(double), hipMemcpyDeviceToHost, s));
+  RP_HIP(hipStreamSynchronize(s));
   return RP_OK;
 }
 

@@ -49,6 +49,7 @@ _SIGS = {
     "rp_pes_destroy": [vp],
     "rp_pes_set_v0": [vp, C.c_double],
     "rp_pes_potforce": [vp, C.c_int, dp, dp, dp],
+    "rp_pes_potforce_noalloc": [vp, C.c_int, dp, dp, dp],
     "rp_pes_hessian": [vp, C.c_int, dp, dp],
     "rp_pes_param_grad": [vp, C.c_int, dp, dp, ip],
     "rp_nm_setup": [vp, C.c_int, C.c_int, C.c_int, dp, C.c_double, C.c_double, C.c_double,
