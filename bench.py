@@ -95,10 +95,34 @@ def plain_sample(pes, sp, ns, steps):
     return out
 
 
+MKL_RT = "/opt/conda/lib/libmkl_rt.so"
+
+
+def host_cores():
+    """The host threads this process may use: its CPU affinity, capped by OMP_NUM_THREADS
+    (which a shared GPU host sets to the job's share)."""
+    cores = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        cores = min(cores, int(env))
+    return cores
+
+
 def cpu_sample(sp, ns, budget_s, threads):
-    """Oracle restatement on the host cores: seconds per polymer-step at each N."""
+    """The oracle's restatement of the reference step on the host cores: seconds per
+    polymer-step at each N.  It runs the -O3 build, with the transforms' mat-vec through
+    MKL's sequential dsymv -- the call the reference makes (verletmodule.f90:259, 279,
+    makefile:6) -- when the MKL runtime is on the host, else through the restatement's loop."""
+    import ctypes
+
     import oracle as O
 
+    blas = None
+    if os.path.exists(MKL_RT):
+        os.environ["MKL_THREADING_LAYER"] = "SEQUENTIAL"
+        os.environ["MKL_NUM_THREADS"] = "1"
+        blas = ctypes.cast(ctypes.CDLL(MKL_RT).dsymv_, ctypes.c_void_p).value
+    O.use_blas_dsymv(blas)
     op = O.Pes(sp.kind, sp.ndim, sp.natom, sp.dp, sp.ip)
     out, detail = {}, {}
     for n in ns:
@@ -108,14 +132,16 @@ def cpu_sample(sp, ns, budget_s, threads):
         x0 = np.outer(sp.well1, 1 - t) + np.outer(b, t)
         # calibrate: one trajectory, few steps
         t1, _ = O.bench_pile(op, n, sp.ndim, sp.natom, sp.mass, betan, TAU, DT, GAMMA, sp.well1,
-                             b, sp.well2 - sp.well1, x0, 1, 4, 1, 1)
+                             b, sp.well2 - sp.well1, x0, 1, 4, 1, 1, fast=True)
         per = t1 / 4
         steps = max(4, int(budget_s / len(ns) / per))
         wall, _ = O.bench_pile(op, n, sp.ndim, sp.natom, sp.mass, betan, TAU, DT, GAMMA,
-                               sp.well1, b, sp.well2 - sp.well1, x0, threads, steps, threads, 1)
+                               sp.well1, b, sp.well2 - sp.well1, x0, threads, steps, threads, 1,
+                               fast=True)
         out[n] = wall / (threads * steps)  # seconds per polymer-step, all cores busy
         detail[n] = {"traj": threads, "steps": steps, "wall_s": round(wall, 3)}
-    return out, detail
+    O.use_blas_dsymv(None)
+    return out, detail, ("mkl_dsymv" if blas else "loop")
 
 
 def copy_probe(mib):
@@ -133,16 +159,22 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 8, "bytes_written": n * 8}))
 
 
-def instanton_sample(pes, sp, nchain=256, reps=10, maxiter=300, seed=4):
+def instanton_sample(pes, sp, nchain=256, reps=10, seed=4):
     """The batched instanton workload on the same EVB surface: nchain open chains with N
     uniform in [64, 1024] and beta in [1000, 8000] (RPI betan = beta/N, rpi_ser.f90:45).
-    Reports bead-evaluations per second of U_M with its adjoint gradient and of the HVP,
-    log-determinant bead blocks per second (lndetJ0 at well 1), and one batched L-BFGS
-    run from straight lines; each figure includes the host copies of its inputs/outputs
-    (the gradient and HVP figures move n*ndof doubles each way per call), except the
-    energy-only U_M figure, which returns one double per chain.  The L-BFGS search keeps
-    its vectors on the device: all chains iterate in lockstep, so its wall time follows
-    the largest iteration count."""
+
+    Throughput: bead-evaluations per second of U_M with its adjoint gradient and of the HVP,
+    log-determinant bead blocks per second (lndetJ0 at well 1); each figure includes the
+    host copies of its inputs/outputs (the gradient and HVP figures move n*ndof doubles each
+    way per call), except the energy-only U_M figure, which returns one double per chain.
+
+    Solution (rpi_ser.f90:95, 146-191, 221-237, 354-381): V0 = V(well1); every chain's
+    instanton from a straight line by the device L-BFGS with the reference's settings
+    (m = 8, factr = 1e6, pgtol = eps2 = 1e-5, no iteration cap, instantonmod.f90:741-767);
+    V0 moved down to the instanton's lowest bead where it dips below V(well1);
+    lndetJ at each instanton, lndetJ0 with every bead at well 1, and the splitting
+    Delta = 2 theta / betan.  Reported: converged chains, iterations, wall time of each stage
+    and in total, and per chain S = betan U_M, lndetJ, lndetJ0 and Delta (cm^-1)."""
     from rptunnel import Ring
 
     rng = np.random.default_rng(seed)
@@ -172,27 +204,61 @@ def instanton_sample(pes, sp, nchain=256, reps=10, maxiter=300, seed=4):
     for _ in range(reps):
         ring.hvp(v)
     t_hvp = (time.perf_counter() - t0) / reps
-    ring.set_x([np.repeat(sp.well1[:, None], n, axis=1) for n in ns])
+    wells = [np.repeat(sp.well1[:, None], n, axis=1) for n in ns]
+    ring.set_x(wells)
     ring.logdet(singlewell=True, skip_lowest=False)
     t0 = time.perf_counter()
     for _ in range(max(1, reps // 5)):
         ring.logdet(singlewell=True, skip_lowest=False)
     t_ld = (time.perf_counter() - t0) / max(1, reps // 5)
+
+    # the solution, as rpi_ser.f90 computes it for every chain
+    pes.set_V0(pes.V(sp.well1))
     t0 = time.perf_counter()
-    _, act, it, st = ring.instanton(chains, pgtol=1e-5, factr=1e6, m=8, maxiter=maxiter)
+    xo, act, it, st = ring.instanton(chains, pgtol=1e-5, factr=1e6, m=8, maxiter=0)
     t_opt = time.perf_counter() - t0
+    # rpi_ser.f90:181-191: when the instanton dips below V(well1), V0 moves to its lowest
+    # bead, which lowers every bead's V - V0 (and U_M by n times the dip)
+    for c, x in enumerate(xo):
+        vmin = float(np.min(pes.V(x)))
+        if vmin < 0.0:
+            act[c] -= ns[c] * vmin
+    ring.set_x(xo)
+    t0 = time.perf_counter()
+    ldj, nneg, _ = ring.logdet(singlewell=False, skip_lowest=True)
+    t_ldj = time.perf_counter() - t0
+    ring.set_x(wells)
+    t0 = time.perf_counter()
+    ldj0, _, _ = ring.logdet(singlewell=True, skip_lowest=False)
+    t_ldj0 = time.perf_counter() - t0
+    pes.set_V0(0.0)
     ring.close()
+    bn = np.asarray(betan)
+    skink = bn * act
+    phi = np.exp(0.5 * (ldj - ldj0))
+    theta = bn * np.exp(-skink) * np.sqrt(skink / (2.0 * 3.14159265358979)) / phi
+    delta_cm = 2.0 * theta / bn * 219475.0
+
+    def r10(a):
+        return [float(f"{float(x):.10g}") for x in a]
+
     return {"workload": f"evb27 instantons: {nchain} chains, N in [64, 1024] (total {beads} beads), "
                         "beta in [1000, 8000]",
             "um_grad_bead_evals_per_s": beads / t_um,
             "um_energy_only_bead_evals_per_s": beads / t_e,
             "hvp_bead_evals_per_s": beads / t_hvp,
             "logdet_bead_blocks_per_s": beads / t_ld,
-            "lbfgs": {"seconds": t_opt, "max_iterations": int(np.max(it)),
-                      "ms_per_iteration": 1e3 * t_opt / max(int(np.max(it)), 1),
-                      "mean_iterations": float(np.mean(it)),
-                      "converged": int(np.sum(st == 0)), "chains": nchain,
-                      "pgtol": 1e-5, "factr": 1e6, "iteration_cap": maxiter}}
+            "solution": {
+                "settings": "V0 = V(well1); L-BFGS m = 8, factr = 1e6, pgtol = 1e-5, no iteration cap",
+                "converged": int(np.sum(st == 0)), "chains": nchain,
+                "status_counts": {str(k): int(np.sum(st == k)) for k in np.unique(st)},
+                "iterations": {"max": int(np.max(it)), "mean": float(np.mean(it))},
+                "seconds": {"lbfgs": t_opt, "lndetJ": t_ldj, "lndetJ0": t_ldj0,
+                            "total": t_opt + t_ldj + t_ldj0},
+                "lbfgs_ms_per_iteration": 1e3 * t_opt / max(int(np.max(it)), 1),
+                "negative_eigenvalues_at_instanton": {str(k): int(np.sum(nneg == k)) for k in np.unique(nneg)},
+                "per_chain": {"n": ns, "beta": r10(betas), "S": r10(skink), "lndetJ": r10(ldj),
+                              "lndetJ0": r10(ldj0), "delta_cm-1": r10(delta_cm)}}}
 
 
 def water_sample(steps=16, warmup=8, ntraj=1024, n=128, nchain=256):
@@ -250,6 +316,68 @@ def water_sample(steps=16, warmup=8, ntraj=1024, n=128, nchain=256):
             "param_grad_bead_evals_per_s": nchain * n / tpg}
 
 
+def equal_splitting_sample(pes, sp, n=32, nrep=4, nsteps=400, imin=100, seed=77):
+    """The metric's "at equal TI splitting error" clause, measured: the same TI tasks
+    (11 Gauss-Legendre nodes x nrep replicas of the 27-dof EVB at one N, init_path, nsteps
+    Langevin steps, TI after imin) on the batched engine and on the plain per-trajectory
+    path, drawing the same Philox streams.  Reports both paths' integrands' largest
+    relative difference, Delta A, sigma_A, q/q0 and the two-level splitting 2 artanh(q/q0)/beta
+    from each, and the plain path's bead-steps/s over all its trajectories."""
+    from rptunnel import Batch, ti
+
+    def factory(pols):
+        return Batch(pes, sp.mass, pols, TAU, DT, GAMMA, seed=seed)
+
+    rb = ti.run_ti(factory, sp, n, BETA, NLAMBDA, nrep, nsteps, imin)
+    rp = ti.run_ti_plain(pes, sp, n, BETA, NLAMBDA, nrep, nsteps, imin, TAU, DT, GAMMA, seed)
+    ib, ip = rb["integrands"], rp["integrands"]
+    diff = float(np.max(np.abs(ib - ip) / np.maximum(np.abs(ip), 1e-300)))
+
+    def summary(r):
+        q = r["q_ratio"]
+        return {"deltaA": r["deltaA"], "sigmaA": r["sigmaA"], "q_ratio": q, "q_err": r["q_err"],
+                "splitting_cm-1": (2.0 * np.arctanh(q) / BETA * 219475.0) if 0.0 < q < 1.0 else None}
+
+    ntraj = NLAMBDA * nrep
+    return {"workload": f"evb27 TI at N = {n}: {NLAMBDA} nodes x {nrep} replicas, {nsteps} steps "
+                        f"(TI after {imin}), same Philox streams on both paths",
+            "max_rel_integrand_difference": diff,
+            "batched": summary(rb), "plain": summary(rp),
+            "plain_bead_steps_per_s": ntraj * n * nsteps / rp["seconds"],
+            "plain_trajectories_timed": ntraj}
+
+
+def andersen_sample(pes, sp, ns=(64, 256), steps=16, ntraj=NTRAJ):
+    """The Andersen thermostat (propagate_pimd_nm, verletmodule.f90:190-250) on the same
+    27-dof EVB batch shape at two bead counts: 512 trajectories x 11 lambda per N, Poisson
+    resampling with mean interval 5 steps, 8 steps per launch.  Bead-steps/s and each
+    family's average launch time."""
+    from rptunnel import Batch
+    from rptunnel.batch import ANDERSEN
+
+    pols, lampath, path = build_polymers(sp, list(ns), ntraj, NLAMBDA)
+    b = Batch(pes, sp.mass, pols, TAU, DT, thermostat=ANDERSEN, noutput=5, seed=2025)
+    b.set_path(lampath, path)
+    b.init_path()
+    b.set_launch(8)
+    b.run(8)
+    rp_sync()
+    t0 = time.perf_counter()
+    b.run(steps)
+    rp_sync()
+    wall = time.perf_counter() - t0
+    ms, _ = b.last_kernel_ms()
+    sums, _, _ = b.ti_readout()
+    assert np.all(np.isfinite(sums)), "non-finite TI sums (Andersen)"
+    fams = [{"n": n, "waves": w, "units_per_wave": u, "avg_launch_ms": round(float(m), 4)}
+            for (n, w, u, _), m in zip(b.family_shapes(), ms)]
+    b.close()
+    return {"workload": f"evb27 batched TI, Andersen (noutput 5): N = {list(ns)}, {ntraj} traj x "
+                        f"{NLAMBDA} lambda each ({len(pols)} polymers)",
+            "bead_steps_per_s": sum(p["n"] for p in pols) * steps / wall,
+            "ms_per_step": 1e3 * wall / steps, "steps_per_launch": 8, "families": fams}
+
+
 def rp_sync():
     import rptunnel
     rptunnel.lib().rp_finalize()  # device synchronize
@@ -266,6 +394,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-instanton", action="store_true",
                     help="skip the batched-instanton sample reported beside the headline")
+    ap.add_argument("--no-andersen", action="store_true",
+                    help="skip the Andersen-thermostat sample reported beside the headline")
     ap.add_argument("--no-water", action="store_true",
                     help="skip the water-cluster sample reported beside the headline")
     ap.add_argument("--steps-per-launch", type=int, default=8,
@@ -341,6 +471,14 @@ def main():
     polys_per_n = args.ntraj * NLAMBDA
     plain_value = sum(n * polys_per_n for n in NS) / sum(tp[n] * polys_per_n for n in NS)
 
+    log("equal-splitting sample: the same TI on the batched and the plain path")
+    eqs = equal_splitting_sample(pes, sp)
+
+    andersen = None
+    if not args.no_andersen:
+        log("Andersen sample (N = 64, 256)")
+        andersen = andersen_sample(pes, sp)
+
     inst = None
     if not args.no_instanton:
         log("instanton sample (256 chains)")
@@ -353,14 +491,24 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cores()
         log(f"cpu baseline: oracle on {threads} threads, ~{args.cpu_seconds:.0f} s")
-        tc, detail = cpu_sample(sp, NS, args.cpu_seconds, threads)
+        tc, detail, symv = cpu_sample(sp, NS, args.cpu_seconds, threads)
         cpu_value = sum(n for n in NS) / sum(tc[n] for n in NS)
+        calib = None
+        cfile = os.path.join(ROOT, "oracle", "cpu_calibration.json")
+        if os.path.exists(cfile):
+            with open(cfile) as f:
+                c = json.load(f)
+            calib = {"restatement_loop_over_mkl_dsymv_mix": round(c["mix_ratio_restatement_over_mkl"], 3),
+                     "per_N": {k: round(v["ratio"], 3) for k, v in c["sec_per_polymer_step"].items()},
+                     "measured_on": f"{c['cpu']}, {c['cores']} cores (oracle/calibrate_cpu.py)"}
         cpu = {"value": cpu_value, "unit": "bead-steps/s", "cores": threads, "kind": "port",
                "sample": "oracle restatement of the reference PILE step (10 dense transforms "
-                         "per step, reference step order), one trajectory per thread, "
-                         f"at every N of the mix: {detail}; equal polymer counts per N"}
+                         f"per step through {'MKL sequential dsymv' if symv == 'mkl_dsymv' else 'the restatement loop'}, "
+                         "reference step order; -O3 build), one trajectory per thread, "
+                         f"at every N of the mix: {detail}; equal polymer counts per N",
+               "symv": symv, "calibration": calib}
 
     out = {
         "metric": METRIC,
@@ -389,6 +537,8 @@ def main():
                      "flops_per_bead_step": flops_per_bead_step(dn, D),
                      "families": fam_rows},
         "cpu_baseline": cpu,
+        "equal_splitting": eqs,
+        "andersen": andersen,
         "instanton": inst,
         "water": water,
     }

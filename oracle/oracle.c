@@ -453,7 +453,23 @@ void or_nm_free(or_nm *nm) {
 }
 
 /* dsymv('U', n, 1, T, n, x, 1, 0, y, 1) -- y = T x, T symmetric */
+/* The transforms' symmetric mat-vec.  The reference calls BLAS dsymv('U', ...)
+ * (verletmodule.f90:259, 279); the restatement loops by default, and
+ * or_use_blas_dsymv swaps in a BLAS dsymv_ (e.g. MKL's) so the CPU baseline can be
+ * calibrated against the library the reference links (makefile:6).  NULL restores the loop. */
+typedef void (*or_dsymv_fn)(const char *uplo, const int *n, const double *alpha, const double *a,
+                            const int *lda, const double *x, const int *incx, const double *beta,
+                            double *y, const int *incy);
+static or_dsymv_fn g_blas_dsymv = NULL;
+void or_use_blas_dsymv(void *f) { g_blas_dsymv = (or_dsymv_fn)f; }
+
 static void symv(int n, const double *T, const double *x, double *y) {
+  if (g_blas_dsymv) {
+    const double one = 1.0, zero = 0.0;
+    const int inc = 1;
+    g_blas_dsymv("U", &n, &one, T, &n, x, &inc, &zero, y, &inc);
+    return;
+  }
   for (int i = 0; i < n; ++i) {
     double s = 0.0;
     for (int l = 0; l < n; ++l) s += T[i * n + l] * x[l];

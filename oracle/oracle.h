@@ -96,6 +96,8 @@ int or_nm_init(or_nm *nm, int n, int ndim, int natom, const double *mass, double
 void or_nm_free(or_nm *nm);
 /* nmtransform_forward / backward on one dof's n-vector (verletmodule.f90:254-286);
  * bead < 0 means "no shift" (the reference's bead = 0). */
+/* swap a BLAS dsymv_ (Fortran ABI, 32-bit integers) into the transforms; NULL = the loop */
+void or_use_blas_dsymv(void *f);
 void or_nm_forward(const or_nm *nm, const double *x, double *q, int dof);
 void or_nm_backward(const or_nm *nm, const double *q, double *x, int dof);
 

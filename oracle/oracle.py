@@ -20,7 +20,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "_build", "liboracle.so")
+_LIB_FAST = os.path.join(_HERE, "_build", "liboracle_fast.so")  # the CPU baseline's timed build
 _lib = None
+_lib_fast = None
 
 PES_QUARTIC, PES_COUPLED2D, PES_EVB, PES_WATER, PES_HARMONIC = 1, 2, 3, 4, 5
 OR_PI = 3.14159265358979  # instantonmod.f90:4
@@ -33,8 +35,16 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _lib
+def lib(fast=False):
+    """The parity build, or (fast=True) the -O3 build the CPU baseline times."""
+    global _lib, _lib_fast
+    if fast:
+        if _lib_fast is None:
+            if not os.path.exists(_LIB_FAST):
+                build()
+            _lib_fast = C.CDLL(_LIB_FAST)
+            _declare(_lib_fast)
+        return _lib_fast
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
@@ -114,6 +124,7 @@ def _declare(L):
         "or_UMprime": (None, [P(OrChain), P(OrPes), _dp, _dp, _dp, _dp]),
         "or_UMforceenergy": (C.c_double, [P(OrChain), P(OrPes), _dp, _dp, _dp, _dp]),
         "or_UMhessian": (None, [P(OrChain), P(OrPes), _dp, C.c_int, _dp]),
+        "or_use_blas_dsymv": (None, [C.c_void_p]),
         "or_bench_pile": (C.c_double, [P(OrPes), C.c_int, C.c_int, C.c_int, _dp, C.c_double,
                                        C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp,
                                        C.c_int, C.c_int64, C.c_int, C.c_uint64, _dp]),
@@ -471,10 +482,16 @@ def splitting_two_beta(Ia, Ib, beta1, beta2):
 
 
 def bench_pile(pes, n, ndim, natom, mass, betan, tau, dt, gamma, a, b, dbdl, x0, ntraj, nsteps,
-               nthreads, seed):
+               nthreads, seed, fast=False):
     """Wall seconds for ntraj PILE trajectories (the CPU baseline) and their TI means."""
     mass, a, b, dbdl, x0 = (_f64(v) for v in (mass, a, b, dbdl, x0))
     out = np.zeros(ntraj)
-    t = lib().or_bench_pile(C.byref(pes.s), n, ndim, natom, _d(mass), betan, tau, dt, gamma,
-                            _d(a), _d(b), _d(dbdl), _d(x0), ntraj, nsteps, nthreads, seed, _d(out))
+    t = lib(fast).or_bench_pile(C.byref(pes.s), n, ndim, natom, _d(mass), betan, tau, dt, gamma,
+                                _d(a), _d(b), _d(dbdl), _d(x0), ntraj, nsteps, nthreads, seed,
+                                _d(out))
     return t, out
+
+
+def use_blas_dsymv(fn_address, fast=True):
+    """Route the transforms' mat-vec through a BLAS dsymv_ at fn_address (None: the loop)."""
+    lib(fast).or_use_blas_dsymv(fn_address)

@@ -144,7 +144,9 @@ static int batch_flags(rp_batch* b) {
 }
 
 // Waves per workgroup by default: 8 (two per SIMD, which hides the latency of the
-// elementwise phases) from 8 to 16 MFMA units, 4 otherwise.  Measured on the 27-dof EVB
+// elementwise phases) from 8 to 16 MFMA units, 4 otherwise.  16 waves (four per SIMD, one
+// unit each) measured slower at every N (8-step launch of 5632 polymers, ms, 8 / 16 waves:
+// N = 128 4.39 / 8.75, 192 7.50 / 11.20, 256 11.31 / 13.59).  Measured on the 27-dof EVB
 // (ms per 8-step launch of 5632 polymers, 4 / 8 waves): 2 units 1.37 / 2.15, 4 units
 // 1.69 / 2.68, 6 units 3.43 / 3.85 (two 4-wave workgroups share a CU), 8 units 5.30 / 4.37,
 // 12 units 8.88 / 7.51, 16 units 13.4 / 11.3.  Four 2-wave workgroups per CU for 2 units (no

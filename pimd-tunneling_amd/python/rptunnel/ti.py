@@ -125,6 +125,50 @@ def run_ti(batch_factory, spec, n, beta, nintegral, nrep, nsteps, imin, path=Non
     return res
 
 
+def run_ti_plain(pes, spec, n, beta, nintegral, nrep, nsteps, imin, tau, dt, gamma, seed,
+                 path=None):
+    """The same TI tasks as run_ti, one trajectory at a time on the plain per-trajectory
+    path (rp_chain), as the reference's task loop runs them (pimd_par.f90:321-381): for each
+    task, init_nm with the node's ends, init_path from the task's Philox stream (seed, k),
+    propagate_pimd_pile with TI after imin steps.  With the batched engine's seed the two
+    paths draw the same numbers, so their integrands agree to rounding.  Returns the
+    quadrature dict plus the integrands and the wall seconds of the trajectories."""
+    import time
+
+    from .specs import arc_lampath, linear_path
+    from .verlet import Chain
+
+    betan = beta / (n + 1)
+    if path is None:
+        lampath, pth = linear_path(spec.well1, spec.well2, npath=2)
+    else:
+        pth = np.asarray(path, dtype=np.float64)
+        lampath = arc_lampath(pth)
+    xi, w = gauleg(0.0, 1.0, nintegral)
+    ends, grads = _spline_nodes(lampath, pth, xi)
+    start = pth[:, 0]
+    integrands = np.zeros(nintegral * nrep)
+    wall = 0.0
+    for k, (i, j) in enumerate(task_table(nintegral, nrep)):
+        if skip_task(ends[i]):
+            continue
+        ch = Chain(pes, n, spec.mass, betan, tau, dt, gamma, start, ends[i])
+        ch.set_path(lampath, pth)
+        ch.set_rng(seed, k)
+        t0 = time.perf_counter()
+        ch.init_path(float(xi[i]))
+        dh, _ = ch.propagate_pimd_pile(nsteps, grads[i], imin=imin, xi=float(xi[i]))
+        wall += time.perf_counter() - t0
+        ch.close()
+        integrands[k] = dh / betan ** 2
+    res = quadrature(integrands, w, nrep, betan)
+    res["integrands"] = integrands
+    res["nodes"] = xi
+    res["weights"] = w
+    res["seconds"] = wall
+    return res
+
+
 def _spline_nodes(lampath, path, xi):
     """splint / splin_grad of every dof at the nodes (instantonmod.f90:499-561);
     path is (ndof, npath)."""

@@ -239,3 +239,41 @@ def test_batch_equals_plain_path():
     xc, _ = ch.get_state()
     assert rel(xg, xc) <= 1e-10
     assert abs(sums[0] / 40 - dh) <= 1e-10 * abs(dh)
+
+
+def test_autotune_keeps_results_and_state():
+    """rp_batch_autotune times launch shapes on the live batch and restores its state: the
+    state, TI sums and stream positions afterwards are those before it, and runs with the
+    configuration it picks give bit for bit the default configuration's states, counts and
+    stream positions (TI sums to rounding: they are grouped per launch)."""
+    sp = specs.evb()
+    pes = Pes(sp)
+    pols = _polymers(sp, [32, 128, 32, 96], xis=[0.2, 0.5, 0.7, 0.9])
+    bs = []
+    for tune in (False, True):
+        b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, seed=31)
+        for k, pol in enumerate(pols):
+            x, p = _state(sp, pol["n"], 80 + k)
+            b.set_state(k, x, p)
+        b.run(5, imin=1)
+        if tune:
+            before = [b.get_state(k) for k in range(len(pols))], b.ti_readout(), b.cursors()
+            steps, waves = b.autotune(trial_steps=8)
+            assert steps >= 1 and waves in (4, 8)
+            after = [b.get_state(k) for k in range(len(pols))], b.ti_readout(), b.cursors()
+            for (x0, p0), (x1, p1) in zip(before[0], after[0]):
+                assert np.array_equal(x0, x1) and np.array_equal(p0, p1)
+            assert all(np.array_equal(u, v) for u, v in zip(before[1], after[1]))
+            assert np.array_equal(before[2], after[2])
+        b.run(19)
+        bs.append(b)
+    for k in range(len(pols)):
+        xa, pa = bs[0].get_state(k)
+        xb, pb = bs[1].get_state(k)
+        assert np.array_equal(xa, xb) and np.array_equal(pa, pb), k
+    (s0, c0, r0), (s1, c1, r1) = bs[0].ti_readout(), bs[1].ti_readout()
+    assert np.array_equal(c0, c1) and np.array_equal(r0, r1)
+    # a launch adds its steps' TI terms to the running sum as one partial sum, so another
+    # number of steps per launch groups the same terms differently (rounding only)
+    assert np.all(np.abs(s0 - s1) <= 1e-13 * np.abs(s0))
+    assert np.array_equal(bs[0].cursors(), bs[1].cursors())
