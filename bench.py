@@ -394,6 +394,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-instanton", action="store_true",
                     help="skip the batched-instanton sample reported beside the headline")
+    ap.add_argument("--no-equal", action="store_true",
+                    help="skip the same-TI-on-both-paths sample reported beside the headline")
     ap.add_argument("--no-andersen", action="store_true",
                     help="skip the Andersen-thermostat sample reported beside the headline")
     ap.add_argument("--no-water", action="store_true",
@@ -471,8 +473,10 @@ def main():
     polys_per_n = args.ntraj * NLAMBDA
     plain_value = sum(n * polys_per_n for n in NS) / sum(tp[n] * polys_per_n for n in NS)
 
-    log("equal-splitting sample: the same TI on the batched and the plain path")
-    eqs = equal_splitting_sample(pes, sp)
+    eqs = None
+    if not args.no_equal:
+        log("equal-splitting sample: the same TI on the batched and the plain path")
+        eqs = equal_splitting_sample(pes, sp)
 
     andersen = None
     if not args.no_andersen:
