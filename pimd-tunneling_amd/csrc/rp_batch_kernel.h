@@ -663,12 +663,16 @@ __global__ void __launch_bounds__(W * 64, (W == 4 && UPW == 1) ? 2 : 1)
   const int64_t step0 = A.step0p ? *A.step0p : A.step0;
   for (int s = 0; s < A.nsteps; ++s) {
     const int64_t ii = step0 + s + 1;  // the reference's step counter
-    int ph[6], nph = 0;
+    // the step's phase list, 4 bits per phase in one register (an indexed array would live
+    // in scratch memory): at most 6 phases
+    uint32_t ph = 0;
+    int nph = 0;
+    auto push = [&](int p) __attribute__((always_inline)) { ph |= (uint32_t)p << (4 * nph++); };
     bool resample = false;
     if (langevin) {
       if (A.noise && cur + need > A.noise_cap) { stop = true; break; }
-      ph[nph++] = PH_FWD_PILE;
-      ph[nph++] = PH_BWD;
+      push(PH_FWD_PILE);
+      push(PH_BWD);
     } else {
       // propagate_pimd_nm (verletmodule.f90:202-234): Poisson-timed resample first
       if (rkick < 0) {
@@ -684,18 +688,18 @@ __global__ void __launch_bounds__(W * 64, (W == 4 && UPW == 1) ? 2 : 1)
           break;
         }
         resample = true;
-        ph[nph++] = PH_RESAMPLE;
+        push(PH_RESAMPLE);
       }
-      ph[nph++] = PH_FWD_A;
-      ph[nph++] = PH_BWD;
-      ph[nph++] = PH_FWD_A;
-      ph[nph++] = PH_BWD;
+      push(PH_FWD_A);
+      push(PH_BWD);
+      push(PH_FWD_A);
+      push(PH_BWD);
     }
     int nbwd = 0;
     bool ti_done = false;
 #pragma unroll 1
     for (int k = 0; k < nph; ++k) {
-      const int p = ph[k];
+      const int p = (int)((ph >> (4 * k)) & 15u);
       // With two or more units per wave, the lane indices are re-derived from an opaque
       // copy in every phase: the per-lane LDS addresses are then recomputed where they are
       // used instead of being hoisted out of the step loop and spilled (measured: 16 % less
@@ -703,7 +707,7 @@ __global__ void __launch_bounds__(W * 64, (W == 4 && UPW == 1) ? 2 : 1)
       // One-unit waves keep the hoisted addresses: the opaque copy removes their few spill
       // slots (254 -> 237 VGPRs on the 27-dof EVB) but costs 4-5 % per launch at N = 32-128.
       int lane_k = lane;
-      if constexpr (UPW >= 2) asm volatile("" : "+v"(lane_k));
+      if constexpr (UPW >= 2 || !LANGEVIN) asm volatile("" : "+v"(lane_k));
       const int m = lane_k & 15, g = lane_k >> 4, tid = wave * 64 + lane_k;
       if (p == PH_FWD_PILE || (p == PH_FWD_A && nbwd == 1)) {
         if constexpr (KIND == PES_EVB) {  // B(dt) on the matrix cores
@@ -775,7 +779,7 @@ __global__ void __launch_bounds__(W * 64, (W == 4 && UPW == 1) ? 2 : 1)
             if (A.noise && cur + need > A.noise_cap) {
               stop = true;
             } else {
-              ph[nph++] = PH_REINIT;  // one more phase: re-initialise the path
+              push(PH_REINIT);  // one more phase: re-initialise the path
               ++nre;
             }
           }
