@@ -7,6 +7,11 @@ device adjoint (rp_ring_um), and the backward of that backward is the device
 forward-over-reverse Hessian-vector product (rp_ring_hvp), so first and second
 derivatives of any scalar built from the actions come from the hand-written kernels.
 Tensors may live on any device; the library takes host buffers and copies them itself.
+
+The Ring is the function's workspace: every evaluation loads x into its device coordinates
+and leaves them there (keep an optimised instanton in a Ring of its own).  The gradient made
+by the forward pass is kept for the backward pass, so a first derivative costs one adjoint
+evaluation in all.
 """
 from __future__ import annotations
 
@@ -27,23 +32,25 @@ class _Action(torch.autograd.Function):
         ring.set_x(ring.unpack(flat))
         g, e = ring.UMforceenergy()
         ctx.ring = ring
+        ctx.g = g
         ctx.save_for_backward(x)
         return torch.as_tensor(e, dtype=x.dtype, device=x.device)
 
     @staticmethod
     def backward(ctx, gout):
         (x,) = ctx.saved_tensors
-        return _ActionGrad.apply(x, gout, ctx.ring), None
+        return _ActionGrad.apply(x, gout, ctx.ring, ctx.g), None
 
 
 class _ActionGrad(torch.autograd.Function):
-    """(x, w) -> sum_c w_c grad U_c(x), laid out like x."""
+    """(x, w) -> sum_c w_c grad U_c(x), laid out like x; g = the gradients at x when the
+    caller already holds them."""
 
     @staticmethod
-    def forward(ctx, x, w, ring):
-        flat = _host(x)
-        ring.set_x(ring.unpack(flat))
-        g, _ = ring.UMforceenergy()
+    def forward(ctx, x, w, ring, g=None):
+        if g is None:
+            ring.set_x(ring.unpack(_host(x)))
+            g, _ = ring.UMforceenergy()
         wh = _host(w)
         out = np.concatenate([wh[c] * g[c].reshape(-1) for c in range(ring.nchain)])
         ctx.ring = ring
@@ -66,7 +73,7 @@ class _ActionGrad(torch.autograd.Function):
         dw = np.array([float(np.dot(ctx.g[c].reshape(-1), vs[c].reshape(-1)))
                        for c in range(ring.nchain)])
         return (torch.as_tensor(dx, dtype=x.dtype, device=x.device),
-                torch.as_tensor(dw, dtype=w.dtype, device=w.device), None)
+                torch.as_tensor(dw, dtype=w.dtype, device=w.device), None, None)
 
 
 def ring_action(x: torch.Tensor, ring: Ring) -> torch.Tensor:

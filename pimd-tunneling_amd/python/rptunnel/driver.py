@@ -323,6 +323,14 @@ def orientation_grid(nml):
     raise ValueError("angular integration needs ndim = 2 or 3")
 
 
+def orientation_ibeta(omega, n, ndim):
+    """The I(beta) factor of one orientation: tanh(omega N), capped at 1 when omega N > 1 on
+    the 3-angle grid (rpi_ser.f90:291-295); the 1-angle grid of ndim = 2 has no cap (:341)."""
+    if ndim == 3 and omega * n > 1.0:
+        return 1.0
+    return math.tanh(omega * n)
+
+
 def rotated_well(well2, angles):
     """rpi_ser.f90:258-261 (eta about x, phi about z, theta about x) and :307 (ndim = 2:
     eta about axis 1 of the plane)."""
@@ -364,6 +372,6 @@ def _run_rpi_angular(deck, pes, opts):
         skink = betan * float(um[c])
         g = math.exp(0.5 * (float(ld[c]) - lnd0))
         omega = betan * math.exp(-skink) * math.sqrt(skink / (2.0 * ti.PI_REF)) / g
-        rows.append(dict(angles=angles, weight=wgt, Ibeta=math.tanh(omega * n),
+        rows.append(dict(angles=angles, weight=wgt, Ibeta=orientation_ibeta(omega, n, nml["ndim"]),
                          lndetj=float(ld[c]), skink=skink, status=int(st[c])))
     return dict(lndetj0=lnd0, rows=rows)

@@ -174,5 +174,5 @@ def test_rpi_angular_rows_equal_single_orientation_runs(tmp_path):
         assert out["lndetj0"] == pytest.approx(ang["lndetj0"], rel=1e-13)
         assert row["skink"] == pytest.approx(out["skink"], rel=1e-10)
         assert row["lndetj"] == pytest.approx(out["lndetj"], rel=1e-10, abs=1e-10)
-        want = math.tanh(out["theta"] * 12)
+        want = driver.orientation_ibeta(out["theta"], 12, deck.nml["ndim"])
         assert row["Ibeta"] == pytest.approx(want, rel=1e-9)

@@ -378,6 +378,15 @@ def test_rpi_deck_linear_chain(tmp_path):
     assert pes.V0 == pytest.approx(0.49, rel=1e-15)
 
 
+def test_orientation_ibeta_cap():
+    # rpi_ser.f90:291-295: the 3-angle rows cap I(beta) at 1 once omega N > 1; the ndim = 2
+    # rows (:341) keep tanh(omega N)
+    assert driver.orientation_ibeta(0.125, 12, 3) == 1.0           # omega N = 1.5
+    assert driver.orientation_ibeta(0.125, 12, 2) == math.tanh(1.5)  # 0.905...
+    assert driver.orientation_ibeta(0.05, 12, 3) == math.tanh(0.05 * 12)
+    assert driver.orientation_ibeta(0.25 / 3, 12, 3) == math.tanh(0.25 / 3 * 12)  # omega N <= 1
+
+
 def test_orientation_grid_and_rotations():
     nml = I.rpidata("&RPIDATA npoints=3, cutoffphi=1.0 /")
     g = driver.orientation_grid(nml)
