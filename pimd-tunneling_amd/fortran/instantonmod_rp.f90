@@ -6,7 +6,8 @@
 !   on the device:  UM, UMprime, UMforceenergy, UMhessian, instanton, detJ
 !   on the host:    get_align, align_atoms, rotate_vec, rotate_atoms (:222-376),
 !                   spline, splint, splin_grad, locate (:397-603), findmiddle, eucliddist,
-!                   read_path (:832-1035)
+!                   read_path (:832-1035); read_hess (:1039-1074) only stops, as readhess
+!                   is not supported
 ! The potential is reached through `module rp_bridge` (a device plugin, or any host plugin
 ! of the reference's API).  Chains use the reference layout x(n, ndim, natom); n, ndim,
 ! natom, ndof, totdof come from mcmod_mass.
@@ -529,6 +530,18 @@ contains
     close(u)
     deallocate(frame)
   end subroutine read_path
+
+  ! read_hess (instantonmod.f90:1039-1074) splines Hessians along the path for the readhess
+  ! initial momenta, which this build does not support: a caller that asks for them stops
+  ! here, as rptunnel's deck reader rejects readhess.  The routine exists so the reference's
+  ! pimd_par.f90 (:202-205) links against this module.
+  subroutine read_hess(npath, path, lampath, splinehess, hesspath)
+    integer, intent(in):: npath
+    double precision, allocatable, intent(in):: path(:,:,:), lampath(:)
+    double precision, allocatable, intent(out):: splinehess(:,:,:), hesspath(:,:,:)
+    write(*,*) "read_hess: readhess is not supported by the device build (npath =", npath, ")"
+    stop
+  end subroutine read_hess
 
   ! The lndetJ sums of rpi_ser.f90 at chain x without forming the spectrum: skip_lowest =
   ! .false. gives lndetJ0's sum over every eta > 0 (:227-236), .true. lndetJ's sum over
