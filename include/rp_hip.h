@@ -203,11 +203,12 @@ int rp_batch_families(rp_batch *b, int *nfam, int *n, int *count);
  * into two half-size products (even n) */
 int rp_batch_family_shape(rp_batch *b, int fam, int *waves, int *units_per_wave, int *folded);
 /* launch shape: steps per launch (>= 1; 0 keeps the current value) and waves per
- * polymer workgroup (4 or 8 for every family; 0 keeps each family's value; 8 applies to
- * families of at most 16 MFMA units, the others keep 4).  rp_batch_autotune times, on the
- * live batch (state saved and restored), 4 and 8 waves per family from each family's own
- * launch timings, then the steps-per-launch candidates; it keeps the fastest and reports
- * the steps and the majority wave count. */
+ * polymer workgroup (2, 4 or 8 for every family; 0 keeps each family's value; 8 applies to
+ * families of at most 16 MFMA units, 2 to families of at most 2 units with at most 2 dof
+ * tiles, the others take 4).  rp_batch_autotune times, on the live batch (state saved and
+ * restored), 2, 4 and 8 waves per family from each family's own launch timings, then the
+ * steps-per-launch candidates; it keeps the fastest and reports the steps and the most
+ * frequent wave count. */
 int rp_batch_set_launch(rp_batch *b, int steps_per_launch, int waves);
 /* rp_batch_run replays each family's block of steps_per_launch steps as a captured HIP
  * graph, between that family's timing events, when it draws from the Philox streams

@@ -153,6 +153,7 @@ static int batch_flags(rp_batch* b) {
 // idle wave) measured 1.45 ms against 1.36 for two 4-wave ones.
 static int default_waves(int DT, int NPAD) {
   const int units = (NPAD >> 5) * DT;
+  if (units <= 2 && DT <= 2) return 2;
   return (units >= 8 && units <= 16) ? 8 : 4;
 }
 
@@ -735,13 +736,25 @@ static bool eight_waves_fit(const rp_batch* b, const Family& f) {
   return (f.NPAD >> 5) * b->DT <= 16;
 }
 
+// 2 waves run a family only when each wave holds one of at most two units (W is a multiple
+// of DT); otherwise such a request falls back to 4
+static bool two_waves_fit(const rp_batch* b, const Family& f) {
+  return b->DT <= 2 && (f.NPAD >> 5) * b->DT <= 2;
+}
+
+static int fitted_waves(const rp_batch* b, const Family& f, int w) {
+  if (w == 8) return eight_waves_fit(b, f) ? 8 : 4;
+  if (w == 2) return two_waves_fit(b, f) ? 2 : 4;
+  return 4;
+}
+
 extern "C" int rp_batch_set_launch(rp_batch* b, int steps_per_launch, int waves) {
   RP_REQUIRE(b && steps_per_launch >= 0, "rp_batch_set_launch: bad arguments");
-  RP_REQUIRE(waves == 0 || waves == 4 || waves == 8,
-             "rp_batch_set_launch: waves per workgroup must be 4 or 8");
+  RP_REQUIRE(waves == 0 || waves == 2 || waves == 4 || waves == 8,
+             "rp_batch_set_launch: waves per workgroup must be 2, 4 or 8");
   if (steps_per_launch > 0) b->steps_per_launch = steps_per_launch;
   if (waves)
-    for (Family& f : b->fam) f.waves = (waves == 8 && eight_waves_fit(b, f)) ? 8 : 4;
+    for (Family& f : b->fam) f.waves = fitted_waves(b, f, waves);
   return RP_OK;
 }
 
@@ -783,8 +796,8 @@ extern "C" int rp_batch_autotune(rp_batch* b, int trial_steps, int* best_steps, 
   std::vector<double> best_ms(nf, 1e300), fam_ms;
   std::vector<int> best_w(nf, 4);
   double sec = 0.0;
-  for (int w : {4, 8}) {
-    for (Family& f : b->fam) f.waves = (w == 8 && eight_waves_fit(b, f)) ? 8 : 4;
+  for (int w : {2, 4, 8}) {
+    for (Family& f : b->fam) f.waves = fitted_waves(b, f, w);
     if ((rc = trial(std::min(b->steps_per_launch, trial_steps), fam_ms, sec))) return rc;
     for (size_t fi = 0; fi < nf; ++fi)
       if (b->fam[fi].waves == w && fam_ms[fi] < best_ms[fi]) {
@@ -792,10 +805,10 @@ extern "C" int rp_batch_autotune(rp_batch* b, int trial_steps, int* best_steps, 
         best_w[fi] = w;
       }
   }
-  int n8 = 0;
+  int nw[9] = {0};
   for (size_t fi = 0; fi < nf; ++fi) {
     b->fam[fi].waves = best_w[fi];
-    n8 += best_w[fi] == 8;
+    ++nw[best_w[fi]];
   }
   // 2. steps per launch
   const int cands[] = {8, 32, 128};
@@ -812,7 +825,8 @@ extern "C" int rp_batch_autotune(rp_batch* b, int trial_steps, int* best_steps, 
   b->steps_done = steps0;
   b->steps_per_launch = bs;
   if (best_steps) *best_steps = bs;
-  if (best_waves) *best_waves = 2 * n8 > (int)nf ? 8 : 4;  // the majority choice
+  if (best_waves)  // the most frequent choice (ties to the wider workgroup)
+    *best_waves = (nw[8] >= nw[4] && nw[8] >= nw[2]) ? 8 : (nw[4] >= nw[2] ? 4 : 2);
   return batch_flags(b);
 }
 

@@ -616,7 +616,7 @@ enum : int { PH_FWD_PILE = 0, PH_FWD_A = 1, PH_BWD = 2, PH_RESAMPLE = 3, PH_REIN
 // One-unit 4-wave kernels (small bead counts, whose workgroups need little LDS) are held to
 // two waves per SIMD so that two polymers share a CU.
 template <int KIND, int MAXD, int DT, int UPW, int W, bool LANGEVIN, bool FOLD>
-__global__ void __launch_bounds__(W * 64, (W == 4 && UPW == 1) ? 2 : 1)
+__global__ void __launch_bounds__(W * 64, (W <= 4 && UPW == 1) ? 2 : 1)
     batch_kernel(BatchArgs A) {
   extern __shared__ double S[];
   __shared__ int s_flag;
@@ -827,6 +827,11 @@ int launch_batch_thermo(const BatchArgs& a, int npoly, hipStream_t s) {
     RP_HIP(hipGetLastError());
     return RP_OK;
   };
+  if constexpr (DT <= 2) {
+    // one unit per wave, four 2-wave workgroups per CU (at most 256 VGPRs each): small
+    // polymers whose 2 units would leave half of a 4-wave workgroup idle
+    if (a.waves == 2 && units <= 2) return go(batch_kernel<KIND, MAXD, DT, 1, 2, LANG, FOLD>, 2);
+  }
   if (a.waves == 8) {  // two waves per SIMD: units spread over 8 waves, at most 2 each
     const int upw = (units + 7) / 8;
     if (upw <= 1) return go(batch_kernel<KIND, MAXD, DT, 1, 8, LANG, FOLD>, 8);

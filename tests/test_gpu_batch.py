@@ -133,16 +133,24 @@ def test_graph_replays_match_direct_launches(thermostat):
 
 @pytest.mark.parametrize("thermostat", [2, 1])
 def test_wave_counts_agree(thermostat):
-    """Both workgroup shapes give bit for bit the same states, TI sums and stream positions:
-    4 waves and 8 (families of at most 16 MFMA units, here all): idle unit slots at N = 24,
-    32, 33 and 64, and two units per wave at N = 128 with 4 waves."""
+    """Every workgroup shape gives bit for bit the same states, TI sums and stream positions:
+    2 waves (families of at most 2 MFMA units: N = 24, 32), 4 waves and 8 (families of at
+    most 16 units, here all): idle unit slots at N = 24, 32, 33 and 64, and two units per
+    wave at N = 128 with 4 waves."""
     sp = specs.evb()
     pes = Pes(sp)
     pols = _polymers(sp, [32, 24, 64, 128, 33], xis=[0.2, 0.5, 0.7, 0.9, 0.4])
+    b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, thermostat=thermostat, noutput=5, seed=13)
+    default = {24: 2, 32: 2, 33: 4, 64: 4, 128: 8}  # units: 2, 2, 4, 4, 8
+    assert all(s[1] == default[n] for (n, _), s in zip(b.families(), b.family_shapes()))
+    b.close()
     res = []
-    for waves in (4, 8):
+    for waves in (4, 8, 2):
         b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, thermostat=thermostat, noutput=5, seed=13)
         b.set_launch(steps_per_launch=8, waves=waves)
+        want = {24: 2, 32: 2, 33: 4, 64: 4, 128: 4} if waves == 2 else None
+        for (n, _), shape in zip(b.families(), b.family_shapes()):
+            assert shape[1] == (want[n] if want else waves), (n, shape)
         for k, pol in enumerate(pols):
             x, p = _state(sp, pol["n"], 60 + k)
             b.set_state(k, x, p)
@@ -259,7 +267,7 @@ def test_autotune_keeps_results_and_state():
         if tune:
             before = [b.get_state(k) for k in range(len(pols))], b.ti_readout(), b.cursors()
             steps, waves = b.autotune(trial_steps=8)
-            assert steps >= 1 and waves in (4, 8)
+            assert steps >= 1 and waves in (2, 4, 8)
             after = [b.get_state(k) for k in range(len(pols))], b.ti_readout(), b.cursors()
             for (x0, p0), (x1, p1) in zip(before[0], after[0]):
                 assert np.array_equal(x0, x1) and np.array_equal(p0, p1)
