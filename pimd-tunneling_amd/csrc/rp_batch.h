@@ -80,13 +80,17 @@ inline int batch_dtiles(const rp_pes* pes) {
   return 2;
 }
 
+// The EVB block as the fused kernel's MFMA kick reads it from LDS, for DC = 16 DT padded
+// dofs: [kappa, 0 | cA | cB | lam | cub (DC each) | WA | WB (DC rows of stride DC + 1)],
+// zero beyond the potential's own stride, so no operand load needs a clamp or a select.
+__host__ __device__ constexpr int evb_kick_lds_size(int DC) { return 2 + 4 * DC + 2 * DC * (DC + 1); }
+
 // Doubles of potential parameters staged in LDS by the fused kernel (the EVB matrices;
 // other potentials read their few parameters from global memory).
 inline int batch_pes_lds(const rp_pes* pes, int N) {
   (void)N;
   if (pes->kind != PES_EVB) return 0;
-  const int M = evb_stride(pes->ndof);
-  return 1 + 2 * (M * M + M) + 2 * M;
+  return evb_kick_lds_size(16 * batch_dtiles(pes));
 }
 
 inline size_t batch_lds_bytes(int npad, int dt_tiles, int pes_lds) {

@@ -458,6 +458,8 @@ __device__ __forceinline__ void water_kick_atoms(const BatchArgs& A, double* S, 
 template <int KIND, int MAXD, int DT, int W>
 __device__ __forceinline__ void pes_kick(const BatchArgs& A, double* S, const double* PL, int tid,
                                          bool& nan) {
+  static_assert(KIND != PES_EVB, "the EVB kick runs on the matrix cores (evb_kick_mfma)");
+  (void)PL;
   if constexpr (KIND == PES_WATER) {
     water_kick_atoms<MAXD / 9, DT, W>(A, S, tid, nan);
     return;
@@ -467,10 +469,7 @@ __device__ __forceinline__ void pes_kick(const BatchArgs& A, double* S, const do
     double xs[MAXD], gs[MAXD];
 #pragma unroll
     for (int d = 0; d < MAXD; ++d) xs[d] = (d < D) ? S[b * LD + d] : 0.0;
-    if constexpr (KIND == PES_EVB)
-      (void)evb_eval_at<MAXD>(PL, xs, gs);  // matrices staged in LDS
-    else
-      (void)eval_kind<KIND, MAXD>(A.pes, xs, gs);
+    (void)eval_kind<KIND, MAXD>(A.pes, xs, gs);
 #pragma unroll
     for (int d = 0; d < MAXD; ++d) {
       if (d < D) {
@@ -482,28 +481,59 @@ __device__ __forceinline__ void pes_kick(const BatchArgs& A, double* S, const do
   }
 }
 
+// The EVB parameter block (evb_eval_at's layout, stride M) -> the kick's LDS layout
+// (evb_kick_lds_size): vectors and matrices zero-padded to DC = 16 DT, matrix rows of
+// stride DC + 1 (the y GEMM reads a column of 16 rows per lane group: an odd stride puts
+// them in distinct banks).
+template <int M, int DT, int W>
+__device__ __forceinline__ void stage_evb_kick(const double* __restrict__ dp, double* PL, int tid) {
+  constexpr int DC = DT * 16, LW = DC + 1, NV = 2 + 4 * DC;
+  const double* WA = dp + 1;
+  const double* cA = WA + M * M;
+  const double* WB = cA + M;
+  const double* cB = WB + M * M;
+  const double* lam = cB + M;
+  const double* cub = lam + M;
+  for (int e = tid; e < evb_kick_lds_size(DC); e += W * 64) {
+    double v = 0.0;
+    if (e == 0) {
+      v = dp[0];
+    } else if (e >= 2 && e < NV) {
+      const int k = (e - 2) / DC, i = (e - 2) - k * DC;
+      const double* vec = k == 0 ? cA : k == 1 ? cB : k == 2 ? lam : cub;
+      if (i < M) v = vec[i];
+    } else if (e >= NV) {
+      const int f = e - NV, w = f / (DC * LW), r = f - w * (DC * LW);
+      const int i = r / LW, j = r - i * LW;
+      if (i < M && j < M) v = (w ? WB : WA)[i * M + j];
+    }
+    PL[e] = v;
+  }
+}
+
 // B for the EVB on the matrix cores.  For a tile of 16 beads, both wells' coordinates
 // y = Wt x - cw are one GEMM (rows: 2 x 16*RT well rows; K: the dofs) and the mixed
 // gradient wa WtA^T hA + wb WtB^T hB is a second GEMM (rows: dofs; K: both wells' rows)
 // whose B operand is the first GEMM's accumulators, scaled in place: item j of row tile
 // rt holds row 16 rt + g + 4 j of bead m, exactly the operand of K-chunk 4 rt + j.
 // The per-bead well energies are reduced over the four lane groups with two xor
-// shuffles.  Tiles are dealt to waves round-robin; padded rows and columns are zero.
+// shuffles.  Tiles are dealt to waves round-robin.  The matrices come zero-padded from
+// stage_evb_kick, so padded rows give y = h = 0 and padded dofs no gradient.
 template <int M, int DT, int W>
 __device__ __forceinline__ void evb_kick_mfma(const BatchArgs& A, double* S, const double* PL,
                                               int wave, int m, int g, bool& nan) {
-  constexpr int RT = (M + 15) / 16;  // row tiles per well (== DT)
+  constexpr int RT = DT;             // row tiles per well
   constexpr int KY = 4 * RT;         // K-chunks of the y GEMM (dofs, padded to 16 RT)
   constexpr int KG = 8 * RT;         // K-chunks of the gradient GEMM (both wells' rows)
+  constexpr int DC = DT * 16, LW = DC + 1;
   const int N = A.N, D = A.D, LD = A.LD;
-  constexpr int DC = DT * 16;
   const double kappa = PL[0];
-  const double* WA = PL + 1;
-  const double* cA = WA + M * M;
-  const double* WB = cA + M;
-  const double* cB = WB + M * M;
-  const double* lam = cB + M;
-  const double* cub = lam + M;
+  const double* cA = PL + 2;
+  const double* cB = cA + DC;
+  const double* lam = cB + DC;
+  const double* cub = lam + DC;
+  const double* WA = cub + DC;
+  const double* WB = WA + DC * LW;
   const int ntile = A.NPAD >> 4;
   for (int t = wave; t < ntile; t += W) {
     const int b0 = t * 16;
@@ -519,11 +549,8 @@ __device__ __forceinline__ void evb_kick_mfma(const BatchArgs& A, double* S, con
       const double bx = S[(b0 + m) * LD + j];
 #pragma unroll
       for (int r = 0; r < 2 * RT; ++r) {
-        // unconditional load at a clamped index, then a select: no branch per operand
-        const int i = (r % RT) * 16 + m;
         const double* Wt = r < RT ? WA : WB;
-        const double w = Wt[min(i, M - 1) * M + min(j, M - 1)];
-        ay[r] = mfma_f64((i < M && j < M) ? w : 0.0, bx, ay[r]);
+        ay[r] = mfma_f64(Wt[((r % RT) * 16 + m) * LW + j], bx, ay[r]);
       }
     }
     // h = lam y + 3 c y^2 and the well energies, per bead m
@@ -533,11 +560,8 @@ __device__ __forceinline__ void evb_kick_mfma(const BatchArgs& A, double* S, con
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) {
         const int i = (r % RT) * 16 + g + 4 * jj;
-        const bool ok = i < M;
-        const int ic = min(i, M - 1);
-        const double cwv = r < RT ? cA[ic] : cB[ic], lv = lam[ic], cv = cub[ic];
-        const double cw = ok ? cwv : 0.0, li = ok ? lv : 0.0, ci = ok ? cv : 0.0;
-        const double y = ay[r][jj] - cw, y2 = y * y;
+        const double li = lam[i], ci = cub[i];
+        const double y = ay[r][jj] - (r < RT ? cA[i] : cB[i]), y2 = y * y;
         const double v = (0.5 * li) * y2 + ci * y2 * y;
         if (r < RT)
           va += v;
@@ -565,11 +589,7 @@ __device__ __forceinline__ void evb_kick_mfma(const BatchArgs& A, double* S, con
       const double* Wt = r < RT ? WA : WB;
       const double bh = ay[r][jj];
 #pragma unroll
-      for (int d = 0; d < DT; ++d) {
-        const int dof = d * 16 + m;
-        const double w = Wt[min(i, M - 1) * M + min(dof, M - 1)];
-        ag[d] = mfma_f64((i < M && dof < M) ? w : 0.0, bh, ag[d]);
-      }
+      for (int d = 0; d < DT; ++d) ag[d] = mfma_f64(Wt[i * LW + d * 16 + m], bh, ag[d]);
     }
     const int bead = b0 + m;
 #pragma unroll
@@ -626,7 +646,7 @@ __global__ void __launch_bounds__(W * 64, (W <= 4 && UPW == 1) ? 2 : 1)
   const int poly = A.first + blockIdx.x;
   const int N = A.N, NPAD = A.NPAD, D = A.D, LD = A.LD;
   double* MT = S + NPAD * LD;
-  double* PL = MT + NPAD * kMODE;  // EVB matrices (evb_eval_at layout), KIND == PES_EVB only
+  double* PL = MT + NPAD * kMODE;  // EVB block (stage_evb_kick's layout), KIND == PES_EVB only
   int64_t* pi = A.pi64 + (size_t)poly * kPI64;
   double* st = A.state + pi[PI_STATE_OFF];
   const double* pa = A.pdata + (size_t)poly * 3 * DC;
@@ -645,8 +665,7 @@ __global__ void __launch_bounds__(W * 64, (W <= 4 && UPW == 1) ? 2 : 1)
     S[i * LD + c] = v;
   }
   for (int e = tid; e < NPAD * kMODE; e += W * 64) MT[e] = A.modes[e];
-  if constexpr (KIND == PES_EVB)
-    for (int e = tid; e < evb_dev_size<MAXD>(); e += W * 64) PL[e] = A.pes.dp[e];
+  if constexpr (KIND == PES_EVB) stage_evb_kick<MAXD, DT, W>(A.pes.dp, PL, tid);
   if (tid == 0) s_flag = 0;
   double tisum = 0.0;
   int64_t ticnt = 0, nre = 0;
@@ -819,7 +838,7 @@ __global__ void __launch_bounds__(W * 64, (W <= 4 && UPW == 1) ? 2 : 1)
 template <int KIND, int MAXD, int DT, bool LANG, bool FOLD>
 int launch_batch_thermo(const BatchArgs& a, int npoly, hipStream_t s) {
   const int units = (a.NPAD >> 5) * DT;
-  const size_t lds = batch_lds_bytes(a.NPAD, DT, KIND == PES_EVB ? evb_dev_size<MAXD>() : 0);
+  const size_t lds = batch_lds_bytes(a.NPAD, DT, KIND == PES_EVB ? evb_kick_lds_size(DT * 16) : 0);
   auto go = [&](auto kern, int waves) -> int {
     RP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
