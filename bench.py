@@ -316,6 +316,46 @@ def water_sample(steps=16, warmup=8, ntraj=1024, n=128, nchain=256):
             "param_grad_bead_evals_per_s": nchain * n / tpg}
 
 
+def coupled2d_sample(steps=16, warmup=8, ntraj=4096, nlam=16, n=128, plain_steps=20):
+    """Config 2: the 2D coupled double well (V0 = 0.005, a = 0.7, w = 0.01, c = 0.3, mass
+    1836), PIMD-TI at N = 128 with ntraj trajectories x nlam lambda nodes (65,536 polymers),
+    Langevin, the headline's beta / tau / dt / gamma, on the batched engine (inputs resident,
+    device-timed) and on the plain per-trajectory path (one trajectory, plain_steps steps)."""
+    from rptunnel import Batch, Pes, specs
+
+    sp = specs.coupled2d()
+    pes = Pes(sp)
+    xi, _ = gauleg(nlam)
+    lampath, path = specs.linear_path(sp.well1, sp.well2, npath=2)
+    pols = [dict(n=n, betan=BETA / (n + 1), xi=float(x), a=sp.well1,
+                 b=sp.well1 + x * (sp.well2 - sp.well1), dbdl=sp.well2 - sp.well1,
+                 stream=il * ntraj + r)
+            for il, x in enumerate(xi) for r in range(ntraj)]
+    b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, seed=5)
+    b.set_path(lampath, path)
+    b.init_path()
+    b.set_launch(8)
+    b.run(warmup)
+    rp_sync()
+    t0 = time.perf_counter()
+    b.run(steps)
+    rp_sync()
+    wall = time.perf_counter() - t0
+    ms, _ = b.last_kernel_ms()
+    shape = b.family_shapes()[0]
+    sums, _, _ = b.ti_readout()
+    assert np.all(np.isfinite(sums)), "non-finite TI sums (coupled 2D)"
+    b.close()
+    tp = plain_sample(pes, sp, [n], plain_steps)[n]
+    batched = len(pols) * n * steps / wall
+    return {"workload": f"coupled 2D double well batched TI: N = {n}, {ntraj} traj x {nlam} "
+                        f"lambda ({len(pols)} polymers), Langevin",
+            "bead_steps_per_s": batched, "ms_per_step": 1e3 * wall / steps,
+            "avg_launch_ms": round(float(ms[0]), 4), "steps_per_launch": 8,
+            "kernel_shape": {"waves": shape[1], "units_per_wave": shape[2], "folded": shape[3]},
+            "plain_gpu_bead_steps_per_s": n / tp, "batched_over_plain": batched * tp / n}
+
+
 def equal_splitting_sample(pes, sp, n=32, nrep=4, nsteps=400, imin=100, seed=77):
     """The metric's "at equal TI splitting error" clause, measured: the same TI tasks
     (11 Gauss-Legendre nodes x nrep replicas of the 27-dof EVB at one N, init_path, nsteps
@@ -398,6 +438,8 @@ def main():
                     help="skip the same-TI-on-both-paths sample reported beside the headline")
     ap.add_argument("--no-andersen", action="store_true",
                     help="skip the Andersen-thermostat sample reported beside the headline")
+    ap.add_argument("--no-config2", action="store_true",
+                    help="skip the coupled-2D (config 2) sample reported beside the headline")
     ap.add_argument("--no-water", action="store_true",
                     help="skip the water-cluster sample reported beside the headline")
     ap.add_argument("--steps-per-launch", type=int, default=8,
@@ -488,6 +530,11 @@ def main():
         log("instanton sample (256 chains)")
         inst = instanton_sample(pes, sp)
 
+    config2 = None
+    if not args.no_config2:
+        log("coupled 2D sample (N = 128, 65536 polymers)")
+        config2 = coupled2d_sample()
+
     water = None
     if not args.no_water:
         log("water-cluster sample (N = 128, 11264 polymers)")
@@ -544,6 +591,7 @@ def main():
         "equal_splitting": eqs,
         "andersen": andersen,
         "instanton": inst,
+        "coupled2d": config2,
         "water": water,
     }
     print(json.dumps(out), flush=True)
