@@ -40,10 +40,14 @@ def _oracle_nm(sp, pol, cayley=False, gamma=GAMMA):
 
 
 @pytest.mark.parametrize("name,ns", [("evb", [32, 64, 96]), ("evb", [128, 40]),
+                                     ("evb4", [64, 32]), ("evb10", [64, 32]),
                                      ("quartic", [64, 17]), ("coupled2d", [128, 32]),
                                      ("water6", [32]), ("water2", [64])])
 def test_batch_pile_injected(name, ns):
-    sp = {"evb": specs.evb, "quartic": specs.quartic, "coupled2d": specs.coupled2d,
+    # evb4 / evb10: the kick's other padded shapes (12 dof in one 16-dof tile, 30 dof in two)
+    sp = {"evb": specs.evb, "evb4": lambda: specs.evb(natom=4),
+          "evb10": lambda: specs.evb(natom=10), "quartic": specs.quartic,
+          "coupled2d": specs.coupled2d,
           "water6": lambda: specs.water_cluster(nmol=6),
           "water2": lambda: specs.water_cluster(nmol=2)}[name]()
     pes = Pes(sp)
