@@ -429,7 +429,7 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--ntraj", type=int, default=NTRAJ)
-    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-instanton", action="store_true",

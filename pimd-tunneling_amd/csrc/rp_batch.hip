@@ -143,6 +143,22 @@ static int batch_flags(rp_batch* b) {
   return RP_OK;
 }
 
+bool rp::batch_supports(const rp_pes* pes, int n) {
+  if (!pes || n < 2 || n > 256) return false;
+  const int D = pes->ndof;
+  bool kind_ok = false;
+  switch (pes->kind) {
+    case PES_EVB: kind_ok = D <= 32; break;
+    case PES_WATER: kind_ok = pes->natom == 3 || pes->natom == 6 || pes->natom == 18; break;
+    case PES_QUARTIC: kind_ok = D <= 32; break;
+    case PES_HARMONIC: kind_ok = D <= 16; break;
+    case PES_COUPLED2D: kind_ok = true; break;
+    default: kind_ok = false;  // host potentials and unknown kinds
+  }
+  const int npad = (n + 31) / 32 * 32;
+  return kind_ok && batch_lds_bytes(npad, batch_dtiles(pes), batch_pes_lds(pes, n)) <= 160 * 1024;
+}
+
 // Waves per workgroup by default: 8 (two per SIMD, which hides the latency of the
 // elementwise phases) from 8 to 16 MFMA units, 4 otherwise.  16 waves (four per SIMD, one
 // unit each) measured slower at every N (8-step launch of 5632 polymers, ms, 8 / 16 waves:

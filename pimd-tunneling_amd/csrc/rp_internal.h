@@ -113,6 +113,9 @@ int host_pes_eval(const rp_pes* pes, int nbead, int ld, const double* x, double*
 int host_pes_hessian(const rp_pes* pes, int nbead, const double* x, double* hess);
 // MAXD bucket a potential needs (16, 32, 64) or 0 when unsupported
 int pes_maxd(const rp_pes* pes);
+// Whether the batched engine runs a polymer of n beads on this potential: a device
+// potential of a supported kind and size, n in 2..256, its workgroup within a CU's LDS.
+bool batch_supports(const rp_pes* pes, int n);
 // row stride of the device EVB layout (the MAXD bucket dispatch_pes picks)
 inline int evb_stride(int D) { return D <= 16 ? 16 : D == 27 ? 27 : 32; }
 }  // namespace rp

@@ -32,6 +32,8 @@ struct rp_chain {
   // path spline (init_path / re-initialisation)
   int npath = 0;
   DevBuf<double> d_lampath, d_path, d_y2;
+  // host copies for the fused propagation (a one-polymer batch): ends a, b and the path
+  std::vector<double> ha, hb, hlampath, hpath;
   uint64_t seed = 0, stream = 0;
   int64_t cursor = 0;   // next normal of the stream
   int64_t kcursor = 0;  // next Poisson interval of the stream (Andersen)
@@ -237,6 +239,8 @@ extern "C" int rp_nm_setup(rp_pes* pes, int n, int ndim, int natom, const double
   c->gamma = gamma;
   c->cayley = cayley;
   c->mass.assign(mass, mass + natom);
+  c->ha.assign(a, a + ndim * natom);
+  c->hb.assign(b, b + ndim * natom);
   nm_host_init(c->nm, n, ndim, natom, mass, betan, tau, dt, gamma, a, b);
   const int D = c->ndof, nd = n * D;
   std::vector<int> shift(2 * D);
@@ -293,6 +297,8 @@ extern "C" int rp_chain_destroy(rp_chain* c) {
 extern "C" int rp_chain_set_ends(rp_chain* c, const double* a, const double* b) {
   RP_REQUIRE(c && a && b, "rp_chain_set_ends: NULL argument");
   nm_host_beadvec(c->nm, c->n, c->ndim, c->natom, c->betan, a, b);
+  c->ha.assign(a, a + c->ndof);
+  c->hb.assign(b, b + c->ndof);
   int rc = c->d_bv.upload(c->nm.bv.data(), (size_t)c->n * c->ndof, c->s);
   if (rc) return rc;
   RP_HIP(hipStreamSynchronize(c->s));
@@ -451,6 +457,8 @@ extern "C" int rp_chain_set_path(rp_chain* c, int npath, const double* lampath,
       (rc = c->d_y2.upload(y2.data(), np, c->s)))
     return rc;
   c->npath = npath;
+  c->hlampath.assign(lampath, lampath + npath);
+  c->hpath.assign(path, path + np);
   RP_HIP(hipStreamSynchronize(c->s));
   return RP_OK;
 }
@@ -465,10 +473,72 @@ extern "C" int rp_init_path(rp_chain* c, double xi, const double* noise) {
   return RP_OK;
 }
 
+// propagate_pimd_pile / propagate_pimd_nm as a one-polymer batch: the fused step kernel
+// keeps the trajectory in one workgroup's LDS for all nmc steps (a launch per 8-step
+// block, replayed from a captured graph), instead of a launch sequence and, with
+// dHdrlimit, a host synchronisation per step.  Same draws, step order and TI rules as the
+// per-step loop below (the batched engine follows the same reference routines); the
+// stream positions come back to the chain.  Used for every device potential the batched
+// engine supports (batch_supports).
+static int propagate_fused(rp_chain* c, int thermostat, int64_t nmc, int64_t imin,
+                           double dhdrlimit, int64_t noutput, const double* dbdl, double xi,
+                           const double* noise, int64_t nnoise, const int32_t* kicks,
+                           int64_t nkicks, double* dhdr, int64_t* nreinit) {
+  const int n = c->n, D = c->ndof;
+  const size_t nd = (size_t)n * D;
+  rp_polymer_desc desc;
+  desc.n = n;
+  desc.betan = c->betan;
+  desc.xi = xi;
+  desc.a = c->ha.data();
+  desc.b = c->hb.data();
+  desc.dbdl = dbdl;
+  desc.stream = c->stream;
+  rp_batch* b = nullptr;
+  int rc = rp_batch_create(c->pes, c->mass.data(), 1, &desc, c->tau, c->dt, c->gamma, thermostat,
+                           noutput, dhdrlimit, c->cayley, c->seed, &b);
+  if (rc) return rc;
+  std::vector<double> x(nd), p(nd);
+  double sum = 0.0;
+  int64_t cnt = 0, nre = 0, cur = c->cursor, kcur = c->kcursor;
+  rc = [&]() -> int {
+    int r;
+    if (c->npath >= 2 &&
+        (r = rp_batch_set_path(b, c->npath, c->hlampath.data(), c->hpath.data())))
+      return r;
+    RP_HIP(hipMemcpyAsync(x.data(), c->d_state.p, nd * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    RP_HIP(hipMemcpyAsync(p.data(), c->d_state.p + nd, nd * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    RP_HIP(hipStreamSynchronize(c->s));
+    if ((r = rp_batch_set_state(b, 0, x.data(), p.data()))) return r;
+    if ((r = rp_batch_set_cursor(b, &cur, &kcur))) return r;
+    if (kicks && (r = rp_batch_set_kicks(b, kicks, nkicks))) return r;
+    if ((r = rp_batch_run(b, nmc, imin, noise, noise ? nnoise : 0))) return r;
+    if ((r = rp_batch_ti_readout(b, &sum, &cnt, &nre))) return r;
+    if ((r = rp_batch_get_state(b, 0, x.data(), p.data()))) return r;
+    if ((r = rp_batch_get_cursor(b, &cur)) || (r = rp_batch_get_kick_cursor(b, &kcur))) return r;
+    RP_HIP(hipMemcpyAsync(c->d_state.p, x.data(), nd * sizeof(double), hipMemcpyHostToDevice, c->s));
+    RP_HIP(hipMemcpyAsync(c->d_state.p + nd, p.data(), nd * sizeof(double), hipMemcpyHostToDevice, c->s));
+    RP_HIP(hipStreamSynchronize(c->s));
+    return RP_OK;
+  }();
+  rp_batch_destroy(b);
+  if (rc) return rc;
+  // injected draws leave the chain's Philox positions where they were, as the loop does
+  if (!noise) c->cursor = cur;
+  if (!kicks && thermostat == RP_THERMO_ANDERSEN) c->kcursor = kcur;
+  *dhdr = nmc > imin ? sum / (double)(nmc - imin) : 0.0;
+  if (nreinit) *nreinit = nre;
+  return RP_OK;
+}
+
 extern "C" int rp_propagate_pile(rp_chain* c, int64_t nmc, int64_t imin, double dhdrlimit,
                                  const double* dbdl, double xi, const double* noise,
                                  int64_t nnoise, double* dhdr, int64_t* nreinit) {
   RP_REQUIRE(c && dbdl && dhdr && nmc >= imin && imin >= 0, "rp_propagate_pile: bad arguments");
+  RP_REQUIRE(!noise || nnoise >= 0, "rp_propagate_pile: negative nnoise");
+  if (batch_supports(c->pes, c->n))
+    return propagate_fused(c, RP_THERMO_LANGEVIN, nmc, imin, dhdrlimit, 1, dbdl, xi, noise, nnoise,
+                           nullptr, 0, dhdr, nreinit);
   const int n = c->n, D = c->ndof;
   const size_t nd = (size_t)n * D;
   int rc;
@@ -524,6 +594,9 @@ extern "C" int rp_propagate_nm(rp_chain* c, int64_t nmc, int64_t imin, int64_t n
                                const int32_t* kicks, int64_t nkicks, double* dhdr) {
   RP_REQUIRE(c && dbdl && dhdr && nmc >= imin && imin >= 0 && noutput > 0,
              "rp_propagate_nm: bad arguments");
+  if (batch_supports(c->pes, c->n))
+    return propagate_fused(c, RP_THERMO_ANDERSEN, nmc, imin, -1.0, noutput, dbdl, 0.0, noise,
+                           nnoise, kicks, nkicks, dhdr, nullptr);
   const int n = c->n, D = c->ndof;
   const size_t nd = (size_t)n * D;
   int rc;
