@@ -151,10 +151,14 @@ def test_time_step_nm():
     assert rel(xg, xo) <= TOL and rel(pg, po) <= TOL
 
 
-@pytest.mark.parametrize("name,n,steps", [("quartic", 64, 200), ("evb", 32, 100)])
+@pytest.mark.parametrize("name,n,steps", [("quartic", 64, 200), ("evb", 32, 100),
+                                          ("quartic", 300, 40)])
 def test_propagate_pile_injected(name, n, steps):
+    """The loop runs on the fused step kernel up to 256 beads and as per-step launch
+    sequences beyond (n = 300, at betan = beta/(n+1) with beta = 2000: betan = 30 at that n
+    diverges in the oracle too)."""
     sp = SPECS[name]()
-    pes, ch, nm, op = _pair(sp, n)
+    pes, ch, nm, op = _pair(sp, n, **({"betan": 2000.0 / (n + 1)} if n > 256 else {}))
     x, p = _state(sp, n, 10)
     dbdl = sp.well2 - sp.well1
     noise = np.random.default_rng(11).standard_normal(steps * n * sp.ndof)
@@ -181,10 +185,10 @@ def test_propagate_pile_philox():
     assert rel(xg, xo) <= 1e-10 and abs(dh - dho) <= 1e-10 * abs(dho)
 
 
-def test_propagate_andersen_injected():
+@pytest.mark.parametrize("n", [64, 300])
+def test_propagate_andersen_injected(n):
     sp = specs.quartic()
-    n = 64
-    pes, ch, nm, op = _pair(sp, n, betan=2000.0 / 65, tau=15.0)
+    pes, ch, nm, op = _pair(sp, n, betan=2000.0 / (n + 1), tau=15.0)
     x, p = _state(sp, n, 13)
     dbdl = sp.well2 - sp.well1
     kicks = np.array([3, 5, 2, 7, 4, 6, 3, 5, 2, 8, 4, 4, 9, 3, 5], dtype=np.int32)
