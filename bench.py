@@ -18,6 +18,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -131,9 +132,15 @@ def cpu_sample(sp, ns, budget_s, threads):
         t = (np.arange(n) + 1.0) / (n + 1.0)
         x0 = np.outer(sp.well1, 1 - t) + np.outer(b, t)
         # calibrate: one trajectory, few steps
-        t1, _ = O.bench_pile(op, n, sp.ndim, sp.natom, sp.mass, betan, TAU, DT, GAMMA, sp.well1,
-                             b, sp.well2 - sp.well1, x0, 1, 4, 1, 1, fast=True)
-        per = t1 / 4
+        # calibrate on one trajectory: double the steps until the set-up no longer dominates
+        cal = 4
+        while True:
+            t1, _ = O.bench_pile(op, n, sp.ndim, sp.natom, sp.mass, betan, TAU, DT, GAMMA,
+                                 sp.well1, b, sp.well2 - sp.well1, x0, 1, cal, 1, 1, fast=True)
+            if t1 > 0.05 * budget_s / len(ns) or cal >= 1 << 20:
+                break
+            cal *= 4
+        per = t1 / cal
         steps = max(4, int(budget_s / len(ns) / per))
         wall, _ = O.bench_pile(op, n, sp.ndim, sp.natom, sp.mass, betan, TAU, DT, GAMMA,
                                sp.well1, b, sp.well2 - sp.well1, x0, threads, steps, threads, 1,
@@ -142,6 +149,136 @@ def cpu_sample(sp, ns, budget_s, threads):
         detail[n] = {"traj": threads, "steps": steps, "wall_s": round(wall, 3)}
     O.use_blas_dsymv(None)
     return out, detail, ("mkl_dsymv" if blas else "loop")
+
+
+def _mkl_dsymv():
+    """Address of MKL's sequential dsymv_ (the reference's transform call), or None."""
+    import ctypes
+
+    if not os.path.exists(MKL_RT):
+        return None
+    os.environ["MKL_THREADING_LAYER"] = "SEQUENTIAL"
+    os.environ["MKL_NUM_THREADS"] = "1"
+    return ctypes.cast(ctypes.CDLL(MKL_RT).dsymv_, ctypes.c_void_p).value
+
+
+def config1_sample(threads, nrep=8, nsteps=100000, imin=1000, n=64):
+    """Config 1 in full: the 1D quartic double well (V0 = 0.005, a = 0.7, m = 1836), N = 64,
+    beta = 2000, 11 Gauss-Legendre nodes x nrep replicas x nsteps Langevin steps, on the GPU
+    (the batched engine: init_path, TI after imin, the reference driver's quadrature) and
+    on the host cores (the oracle's restatement through MKL dsymv, from the straight line
+    with resampled momenta, TI from the first step).  Wall time to solution on each side
+    and both Delta A +- sigma_A, q/q0: equal within their error bars."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle as O
+    from rptunnel import Batch, Pes, specs, ti
+
+    sp = specs.quartic()
+    pes = Pes(sp)
+    betan = BETA / (n + 1)
+
+    def factory(pols):
+        b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, seed=11)
+        b.set_launch(64)
+        return b
+
+    rp_sync()
+    t0 = time.perf_counter()
+    g = ti.run_ti(factory, sp, n, BETA, NLAMBDA, nrep, nsteps, imin)
+    rp_sync()
+    tg = time.perf_counter() - t0
+    ntask = int(np.count_nonzero(g["integrands"]))
+    # host cores: the same tasks on the oracle
+    O.use_blas_dsymv(_mkl_dsymv())
+    op = O.Pes(sp.kind, sp.ndim, sp.natom, sp.dp, sp.ip)
+    lampath, path = specs.linear_path(sp.well1, sp.well2, npath=2)
+    xi, w = ti.gauleg(0.0, 1.0, NLAMBDA)
+    ends, grads = ti._spline_nodes(lampath, path, xi)
+    t = (np.arange(n) + 1.0) / (n + 1.0)
+    per_call = max(1, min(nrep, threads))
+
+    def node(i):
+        if ti.skip_task(ends[i]):
+            return np.zeros(nrep)
+        x0 = np.outer(sp.well1, 1 - t) + np.outer(ends[i], t)
+        _, dh = O.bench_pile(op, n, sp.ndim, sp.natom, sp.mass, betan, TAU, DT, GAMMA, sp.well1,
+                             ends[i], grads[i], x0, nrep, nsteps, per_call, 1000 + i, fast=True)
+        return dh / betan ** 2
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=max(1, threads // per_call)) as ex:
+        cpu_int = np.concatenate(list(ex.map(node, range(NLAMBDA))))
+    tc = time.perf_counter() - t0
+    O.use_blas_dsymv(None)
+    c = ti.quadrature(cpu_int, w, nrep, betan)
+    keys = ("deltaA", "sigmaA", "q_ratio", "q_err")
+    return {"workload": f"1D quartic double well TI in full: N = {n}, beta = {BETA:g}, "
+                        f"{NLAMBDA} lambda x {nrep} replicas x {nsteps} steps ({ntask} trajectories; "
+                        "the middle node has b = 0 and is skipped, pimd_par.f90:322-325)",
+            "gpu": {"wall_s": tg, "bead_steps_per_s": ntask * n * nsteps / tg,
+                    **{k: g[k] for k in keys}},
+            "cpu": {"wall_s": tc, "threads": threads,
+                    "bead_steps_per_s": ntask * n * nsteps / tc, **{k: c[k] for k in keys}},
+            "deltaA_difference_in_sigma": abs(g["deltaA"] - c["deltaA"])
+                                          / math.sqrt(g["sigmaA"] / nrep + c["sigmaA"] / nrep)}
+
+
+def cpu_configs_sample(threads, budget_s=3.0):
+    """The host-core baselines of the other configurations (oracle restatement; MKL dsymv
+    for the transforms): config 2 and config 5 bead-steps/s on a bounded window of all
+    cores (as the config-3 baseline), and config 4's instanton figures at N = 64, 256, 1024:
+    U_M + gradient bead-evaluations/s (one chain per thread), the UMhessian band builds and
+    the DSBEVD eigenvalue solve (LAPACK through scipy's eig_banded, the call detJ makes)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import scipy.linalg
+
+    import oracle as O
+    from rptunnel import specs
+
+    out = {}
+    for name, sp in (("config2_coupled2d", specs.coupled2d()),
+                     ("config5_water6", specs.water_cluster(nmol=6))):
+        tc, detail, symv = cpu_sample(sp, [128], budget_s, threads)
+        out[name] = {"bead_steps_per_s": 128 / tc[128], "sample": detail[128], "symv": symv}
+    sp = specs.evb()
+    op = O.Pes(sp.kind, sp.ndim, sp.natom, sp.dp, sp.ip)
+    inst = {}
+    for n in (64, 256, 1024):
+        betan = 4000.0 / n
+        ch = O.Chain(n, sp.ndim, sp.natom, sp.mass, betan, True)
+        k = np.arange(1, n + 1)
+        x = (np.outer(sp.well1, n - k) + np.outer(sp.well2, k - 1)) / (n - 1)
+        ch.UMforceenergy(op, x, sp.well1, sp.well2)
+        t1 = time.perf_counter()
+        ch.UMforceenergy(op, x, sp.well1, sp.well2)
+        per = max(time.perf_counter() - t1, 1e-6)
+        reps = max(1, int(budget_s / 3 / per))
+
+        def work(_):
+            for _ in range(reps):
+                ch.UMforceenergy(op, x, sp.well1, sp.well2)
+
+        t1 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            list(ex.map(work, range(threads)))
+        tu = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        band = ch.UMhessian(op, x, False)
+        th = time.perf_counter() - t1
+        te = None
+        if n <= 256:  # the O((nD)^2) eigensolve takes over a minute at N = 1024
+            t1 = time.perf_counter()
+            scipy.linalg.eig_banded(band, lower=True, eigvals_only=True)
+            te = time.perf_counter() - t1
+        inst[n] = {"um_grad_bead_evals_per_s": threads * reps * n / tu,
+                   "hessian_band_bead_blocks_per_s": n / th,
+                   "dsbevd_bead_blocks_per_s": n / te if te else None}
+    out["config4_evb_instantons"] = {"per_N": inst, "threads_for_um": threads,
+                                     "hessian_and_eigensolve": "one core each; the eigensolve "
+                                     "only up to N = 256"}
+    return out
 
 
 def copy_probe(mib):
@@ -438,6 +575,8 @@ def main():
                     help="skip the same-TI-on-both-paths sample reported beside the headline")
     ap.add_argument("--no-andersen", action="store_true",
                     help="skip the Andersen-thermostat sample reported beside the headline")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip config 1 in full and the host baselines of configs 2, 4 and 5")
     ap.add_argument("--no-config2", action="store_true",
                     help="skip the coupled-2D (config 2) sample reported beside the headline")
     ap.add_argument("--no-water", action="store_true",
@@ -540,6 +679,14 @@ def main():
         log("water-cluster sample (N = 128, 11264 polymers)")
         water = water_sample()
 
+    config1 = cpu_configs = None
+    if not args.no_cpu and not args.no_configs:
+        threads = host_cores()
+        log(f"config 1 in full on the GPU and on {threads} host threads")
+        config1 = config1_sample(threads)
+        log("host baselines of configs 2, 4 and 5")
+        cpu_configs = cpu_configs_sample(threads)
+
     cpu = None
     if not args.no_cpu:
         threads = host_cores()
@@ -592,6 +739,8 @@ def main():
         "andersen": andersen,
         "instanton": inst,
         "coupled2d": config2,
+        "config1_full": config1,
+        "cpu_configs": cpu_configs,
         "water": water,
     }
     print(json.dumps(out), flush=True)
