@@ -354,6 +354,13 @@ def instanton_sample(pes, sp, nchain=256, reps=10, seed=4):
     t0 = time.perf_counter()
     xo, act, it, st = ring.instanton(chains, pgtol=1e-5, factr=1e6, m=8, maxiter=0)
     t_opt = time.perf_counter() - t0
+    # the same search preconditioned by the springs (rp_ring_set_preconditioner), same tests
+    ring.set_preconditioner(0.05)
+    t0 = time.perf_counter()
+    xp, actp, itp, stp = ring.instanton(chains, pgtol=1e-5, factr=1e6, m=8, maxiter=0)
+    t_pc = time.perf_counter() - t0
+    ring.set_preconditioner(0.0)
+    pc_ds = np.asarray(betan) * (actp - act)  # S = betan U_M of the two searches
     # rpi_ser.f90:181-191: when the instanton dips below V(well1), V0 moves to its lowest
     # bead, which lowers every bead's V - V0 (and U_M by n times the dip)
     for c, x in enumerate(xo):
@@ -393,6 +400,11 @@ def instanton_sample(pes, sp, nchain=256, reps=10, seed=4):
                 "seconds": {"lbfgs": t_opt, "lndetJ": t_ldj, "lndetJ0": t_ldj0,
                             "total": t_opt + t_ldj + t_ldj0},
                 "lbfgs_ms_per_iteration": 1e3 * t_opt / max(int(np.max(it)), 1),
+                "preconditioned": {"sigma": 0.05, "converged": int(np.sum(stp == 0)),
+                                   "iterations": {"max": int(np.max(itp)), "mean": float(np.mean(itp))},
+                                   "seconds_lbfgs": t_pc,
+                                   "max_abs_S_difference": float(np.max(np.abs(pc_ds))),
+                                   "chains_lower_S_than_plain": int(np.sum(pc_ds < 0.0))},
                 "negative_eigenvalues_at_instanton": {str(k): int(np.sum(nneg == k)) for k in np.unique(nneg)},
                 "per_chain": {"n": ns, "beta": r10(betas), "S": r10(skink), "lndetJ": r10(ldj),
                               "lndetJ0": r10(ldj0), "delta_cm-1": r10(delta_cm)}}}

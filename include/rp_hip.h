@@ -254,6 +254,15 @@ int rp_ring_logdet(rp_ring *r, int singlewell, int skip_lowest, double *logdet_p
  * the block LDL^T of H - sigma I to a few ulps: the head of the ascending spectrum DSBEVD
  * returns in detJ (instantonmod.f90:782-827) */
 int rp_ring_lowest_eigs(rp_ring *r, int singlewell, int k, double *eigs);
+/* An option of rp_instanton with no counterpart in the reference: L-BFGS with the
+ * initial inverse Hessian gamma P^-1 instead of gamma I, where P is the spring part of the
+ * U_M Hessian per dof column, (m/betan^2) tridiag(-1, 2, -1) + sigma I (one-sided end rows
+ * for free ends), applied by a tridiagonal solve, and gamma = s.y / y.P^-1.y from the
+ * newest correction pair.  The springs set the stiff end of the spectrum (the iteration
+ * count of the plain method grows with N and beta); sigma stands in for the potential's
+ * curvature.  Same stopping tests, same minimum.  sigma <= 0 switches it off (the default:
+ * the reference's method). */
+int rp_ring_set_preconditioner(rp_ring *r, double sigma);
 /* instanton (:679-777): L-BFGS-B (m <= 32 corrections, unbounded as nbd=0) on every chain
  * at once, resident on the device (iterates, gradients and correction pairs stay in HBM;
  * gradients from the adjoint).  x_inout as rp_ring_set_x; action[c] = U_M at the optimum;

@@ -69,6 +69,9 @@ struct LbArgs {
   LbChain* st;
   double* rho;                                // 1/(s.y) per chain and slot, (kMaxM + 1) each
   int* status;                                // status word per chain for the host
+  const double* pctab;                        // preconditioner (ring_pc), or nullptr
+  const int64_t* pcoff;
+  int ndim;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -370,6 +373,194 @@ __global__ void __launch_bounds__(kLbThreads) lb_direction_kernel(LbArgs A) {
   }
 }
 
+// v <- P^-1 v for one chain's vector in HBM (dof columns of n beads, the ring layout), P the
+// spring tridiagonal of rp_ring_set_preconditioner.  A wave solves one dof column: each lane
+// takes a run of at most kPcRun consecutive beads, and both Thomas sweeps -- the forward
+// d'_i = alpha_i d'_{i-1} + inv_i r_i and the backward x_i = alpha_i x_{i+1} + d'_i -- are
+// affine recurrences, so a lane composes its run into one affine map, a wave scan of those
+// maps (xor-free shuffles up / down) gives each lane its incoming value, and the lane
+// replays its run.  The callers bracket it with barriers.
+constexpr int kPcRun = 16;  // beads per lane: n <= 64 kPcRun = 1024
+__device__ void pc_solve(double* v, int n, int D, int ndim, const double* __restrict__ tab) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int run = (n + 63) >> 6;
+  const int i0 = lane * run;
+  for (int col = wave; col < D; col += kLbWaves) {
+    const double* t = tab + (size_t)(col / ndim) * n * 2;
+    double* vc = v + (size_t)col * n;
+    double w[kPcRun];
+    // forward sweep
+    double A = 1.0, B = 0.0;
+#pragma unroll
+    for (int q = 0; q < kPcRun; ++q) {
+      const int i = i0 + q;
+      w[q] = 0.0;
+      if (q < run && i < n) {
+        const double al = t[2 * i + 1];
+        w[q] = vc[i];
+        B = al * B + t[2 * i] * w[q];
+        A = al * A;
+      }
+    }
+    for (int o = 1; o < 64; o <<= 1) {  // inclusive scan: lanes below come first
+      const double A2 = __shfl_up(A, o), B2 = __shfl_up(B, o);
+      if (lane >= o) {
+        B = A * B2 + B;
+        A = A * A2;
+      }
+    }
+    double x = __shfl_up(B, 1);
+    if (lane == 0) x = 0.0;
+#pragma unroll
+    for (int q = 0; q < kPcRun; ++q) {
+      const int i = i0 + q;
+      if (q < run && i < n) {
+        x = t[2 * i + 1] * x + t[2 * i] * w[q];
+        w[q] = x;  // d'_i
+      }
+    }
+    // backward sweep
+    A = 1.0;
+    B = 0.0;
+#pragma unroll
+    for (int q = kPcRun - 1; q >= 0; --q) {
+      const int i = i0 + q;
+      if (q < run && i < n) {
+        const double al = t[2 * i + 1];
+        B = al * B + w[q];
+        A = al * A;
+      }
+    }
+    for (int o = 1; o < 64; o <<= 1) {  // inclusive scan: lanes above come first
+      const double A2 = __shfl_down(A, o), B2 = __shfl_down(B, o);
+      if (lane + o < 64) {
+        B = A * B2 + B;
+        A = A * A2;
+      }
+    }
+    x = __shfl_down(B, 1);
+    if (lane == 63) x = 0.0;
+#pragma unroll
+    for (int q = kPcRun - 1; q >= 0; --q) {
+      const int i = i0 + q;
+      if (q < run && i < n) {
+        x = t[2 * i + 1] * x + w[q];
+        vc[i] = x;
+      }
+    }
+  }
+}
+
+// The direction of the preconditioned search: d = -H g by the two-loop recursion with the
+// initial matrix gamma P^-1 (gamma = s.y / y.P^-1.y of the newest pair; the first and the
+// reset directions are -P^-1 g, tried at unit step).  Plain passes over HBM: with the
+// preconditioner a search takes tens of iterations where the unpreconditioned one takes
+// thousands, so this kernel is not the one tuned for bandwidth.  XT serves as the scratch
+// vector of the y.P^-1.y solve before it receives the trial point.
+__global__ void __launch_bounds__(kLbThreads) lb_direction_pc_kernel(LbArgs A) {
+  __shared__ double red[kLbWaves];
+  __shared__ double alpha[kMaxM + 1];
+  const int c = blockIdx.x;
+  if (A.st[c].op != OP_DIRECTION) return;  // uniform over the workgroup
+  const int k = A.st[c].count, head = A.st[c].head;
+  const double* rho = A.rho + (int64_t)c * (kMaxM + 1);
+  const int64_t base = A.off[c], tot = A.tot;
+  const int n = A.n[c], L = n * A.D;
+  const int m1 = A.m + 1;
+  const double* tab = A.pctab + A.pcoff[c];
+  double* D = A.Dv + base;
+  const double* G = A.G + base;
+  const double* X = A.X + base;
+  double* XT = A.XT + base;
+  const double* S0 = A.S + base;
+  const double* Y0 = A.Y + base;
+  auto slot = [=](int i) { return (head + i) % m1; };
+  auto Sv = [=](int i) { return S0 + (int64_t)slot(i) * tot; };
+  auto Yv = [=](int i) { return Y0 + (int64_t)slot(i) * tot; };
+  // d = -P^-1 g: the first direction and the fallback of a non-descent one
+  auto steepest = [&]() -> double {
+    for (int j = threadIdx.x; j < L; j += kLbThreads) D[j] = -G[j];
+    __syncthreads();
+    pc_solve(D, n, A.D, A.ndim, tab);
+    __syncthreads();
+    double part = 0.0;
+    for (int j = threadIdx.x; j < L; j += kLbThreads) {
+      part += G[j] * D[j];
+      XT[j] = X[j] + D[j];
+    }
+    return lb_sum(part, red);
+  };
+  double dg0;
+  bool reset = false;
+  if (k == 0) {
+    dg0 = steepest();
+  } else {
+    // backward loop: q = g - sum alpha_i y_i, q in D
+    for (int j = threadIdx.x; j < L; j += kLbThreads) D[j] = G[j];
+    for (int i = k - 1; i >= 0; --i) {
+      const double* Si = Sv(i);
+      const double* Yi = Yv(i);
+      double part = 0.0;
+      for (int j = threadIdx.x; j < L; j += kLbThreads) part += Si[j] * D[j];
+      const double al = rho[slot(i)] * lb_sum(part, red);
+      if (threadIdx.x == 0) alpha[i] = al;
+      for (int j = threadIdx.x; j < L; j += kLbThreads) D[j] -= al * Yi[j];
+    }
+    // gamma = s.y / y.P^-1.y of the newest pair, with XT as the solve's scratch
+    const double* Yk = Yv(k - 1);
+    for (int j = threadIdx.x; j < L; j += kLbThreads) XT[j] = Yk[j];
+    __syncthreads();
+    pc_solve(XT, n, A.D, A.ndim, tab);
+    pc_solve(D, n, A.D, A.ndim, tab);
+    __syncthreads();
+    double part = 0.0;
+    for (int j = threadIdx.x; j < L; j += kLbThreads) part += Yk[j] * XT[j];
+    const double ypy = lb_sum(part, red);
+    const double gam = (1.0 / rho[slot(k - 1)]) / ypy;
+    for (int j = threadIdx.x; j < L; j += kLbThreads) D[j] *= gam;
+    // forward loop: r += s_i (alpha_i - rho_i y_i.r)
+    for (int i = 0; i < k; ++i) {
+      const double* Si = Sv(i);
+      const double* Yi = Yv(i);
+      part = 0.0;
+      for (int j = threadIdx.x; j < L; j += kLbThreads) part += Yi[j] * D[j];
+      const double coef = alpha[i] - rho[slot(i)] * lb_sum(part, red);
+      for (int j = threadIdx.x; j < L; j += kLbThreads) D[j] += coef * Si[j];
+    }
+    part = 0.0;
+    for (int j = threadIdx.x; j < L; j += kLbThreads) {
+      const double d = -D[j];
+      D[j] = d;
+      part += G[j] * d;
+      XT[j] = X[j] + d;
+    }
+    dg0 = lb_sum(part, red);
+    reset = !(dg0 < 0.0);  // not a descent direction: the preconditioned steepest descent
+    if (reset) {
+      __syncthreads();
+      dg0 = steepest();
+    }
+  }
+  if (threadIdx.x == 0) {
+    LbChain t = A.st[c];
+    if (reset) {
+      t.count = 0;
+      t.head = 0;
+    }
+    t.dg0 = dg0;
+    t.f0 = t.f;
+    t.phase = LS_BRACKET;
+    t.a_prev = 0.0;
+    t.f_prev = t.f;
+    t.dg_prev = dg0;
+    t.prev_is_g = 1;
+    t.nls = 0;
+    t.a_cur = 1.0;
+    t.op = OP_NONE;
+    A.st[c] = t;
+  }
+}
+
 // one line-search evaluation: the value and slope at the trial point decide the next trial
 // point or the acceptance, which stores the correction pair and tests convergence
 __global__ void __launch_bounds__(kLbThreads) lb_feed_kernel(LbArgs A) {
@@ -556,6 +747,10 @@ extern "C" int rp_instanton(rp_ring* r, double pgtol, double factr, int m, int m
   A.st = st.p;
   A.rho = rho.p;
   A.status = stat.p;
+  const RingPc pc = ring_pc(r);
+  A.pctab = pc.tab;
+  A.pcoff = pc.off;
+  A.ndim = pc.ndim;
   // initial function and gradient (instantonmod.f90:725-739)
   if ((rc = ring_upload_x(r, x_inout)) || (rc = ring_eval(r, true))) return rc;
   lb_init_kernel<<<nc, kLbThreads, 0, rb.s>>>(A);
@@ -563,8 +758,11 @@ extern "C" int rp_instanton(rp_ring* r, double pgtol, double factr, int m, int m
   // direction kernel by the longest chain's elements per thread
   const int nmax = *std::max_element(ring_n(r).begin(), ring_n(r).end());
   const int per = (nmax * rb.D + kLbThreads - 1) / kLbThreads;
-  auto direction = per <= 8 ? lb_direction_kernel<8> : per <= 16 ? lb_direction_kernel<16>
+  auto direction = A.pctab ? lb_direction_pc_kernel
+                 : per <= 8 ? lb_direction_kernel<8> : per <= 16 ? lb_direction_kernel<16>
                  : per <= 28 ? lb_direction_kernel<28> : lb_direction_kernel<0>;
+  RP_REQUIRE(!A.pctab || nmax <= 64 * kPcRun,
+             "rp_instanton: the preconditioner takes chains of at most %d beads", 64 * kPcRun);
   direction<<<nc, kLbThreads, 0, rb.s>>>(A);
   RP_HIP(hipGetLastError());
   std::vector<int> hs(nc);

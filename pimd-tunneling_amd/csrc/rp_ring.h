@@ -26,4 +26,12 @@ struct RingBufs {
   hipStream_t s;
 };
 RingBufs ring_bufs(rp_ring* r);
+// the instanton search's preconditioner (rp_ring_set_preconditioner): Thomas coefficients
+// {1/denominator, alpha} per (chain, atom, bead) from tab + off[chain], or tab = nullptr (off)
+struct RingPc {
+  const double* tab;
+  const int64_t* off;
+  int ndim;
+};
+RingPc ring_pc(rp_ring* r);
 }  // namespace rp

@@ -740,6 +740,11 @@ struct rp_ring {
   // a potential that lives on the host: bead energies on the device, host staging
   DevBuf<double> d_pe;
   std::vector<double> hx, hg, he;
+  // the instanton search's optional preconditioner (rp_ring_set_preconditioner)
+  std::vector<double> hmass;  // per atom
+  double pc_sigma = 0.0;
+  DevBuf<double> d_pctab;
+  DevBuf<int64_t> d_pcoff;
 };
 
 static RingArgs ring_args(rp_ring* r) {
@@ -814,6 +819,9 @@ const std::vector<int>& ring_n(const rp_ring* r) { return r->n; }
 const std::vector<int64_t>& ring_off(const rp_ring* r) { return r->off; }
 int ring_D(const rp_ring* r) { return r->D; }
 int ring_upload_x(rp_ring* r, const double* x) { return r->d_x.upload(x, r->tot, r->s); }
+RingPc ring_pc(rp_ring* r) {
+  return RingPc{r->pc_sigma > 0.0 ? r->d_pctab.p : nullptr, r->d_pcoff.p, r->ndim};
+}
 RingBufs ring_bufs(rp_ring* r) {
   return RingBufs{r->d_x.p, r->d_g.p, r->d_e.p, r->d_off.p, r->d_n.p, r->D, r->s};
 }
@@ -859,6 +867,7 @@ extern "C" int rp_ring_create(rp_pes* pes, const double* mass, int nchain, const
     if (a) memcpy(ab.data() + (size_t)c * 2 * D, a + (size_t)c * D, D * sizeof(double));
     if (b) memcpy(ab.data() + (size_t)c * 2 * D + D, b + (size_t)c * D, D * sizeof(double));
   }
+  r->hmass.assign(mass, mass + r->natom);
   std::vector<double> massd(D);
   for (int at = 0; at < r->natom; ++at)
     for (int j = 0; j < r->ndim; ++j) massd[at * r->ndim + j] = mass[at];
@@ -882,6 +891,45 @@ extern "C" int rp_ring_create(rp_pes* pes, const double* mass, int nchain, const
     return rc;
   }
   *out = r;
+  return RP_OK;
+}
+
+// The spring part of the U_M Hessian per dof column, P = k tridiag(-1, 2, -1) + sigma I with
+// k = m / betan^2 (instantonmod.f90:50-89; free ends: one-sided end rows, diagonal k), as
+// Thomas-algorithm coefficients per (chain, atom, bead): {1/denominator, -off/denominator}.
+extern "C" int rp_ring_set_preconditioner(rp_ring* r, double sigma) {
+  RP_REQUIRE(r, "rp_ring_set_preconditioner: NULL ring");
+  RP_REQUIRE(sigma == sigma, "rp_ring_set_preconditioner: sigma is NaN");
+  r->pc_sigma = sigma > 0.0 ? sigma : 0.0;
+  if (r->pc_sigma == 0.0) return RP_OK;
+  std::vector<int64_t> pcoff(r->nchain);
+  int64_t tot = 0;
+  for (int c = 0; c < r->nchain; ++c) {
+    pcoff[c] = tot;
+    tot += (int64_t)r->natom * r->n[c] * 2;
+  }
+  std::vector<double> tab(tot);
+  for (int c = 0; c < r->nchain; ++c) {
+    const int n = r->n[c];
+    for (int at = 0; at < r->natom; ++at) {
+      const double k = r->hmass[at] / (r->betan[c] * r->betan[c]), off = -k;
+      double* t = tab.data() + pcoff[c] + (int64_t)at * n * 2;
+      double cp = 0.0;  // c'_{i-1}
+      for (int i = 0; i < n; ++i) {
+        const bool end = !r->fixedends && (i == 0 || i == n - 1);
+        const double diag = (end ? k : 2.0 * k) + r->pc_sigma;
+        const double inv = 1.0 / (diag - off * cp);
+        cp = off * inv;
+        t[2 * i] = inv;
+        t[2 * i + 1] = -cp;
+      }
+    }
+  }
+  int rc;
+  if ((rc = r->d_pctab.alloc(tot)) || (rc = r->d_pctab.upload(tab.data(), tot, r->s)) ||
+      (rc = r->d_pcoff.alloc(r->nchain)) || (rc = r->d_pcoff.upload(pcoff.data(), r->nchain, r->s)))
+    return rc;
+  RP_HIP(hipStreamSynchronize(r->s));
   return RP_OK;
 }
 

@@ -380,6 +380,12 @@ module rp_hip
        integer(c_int), value :: singlewell, k
        real(c_double), intent(out) :: eigs(*)
      end function rp_ring_lowest_eigs
+     integer(c_int) function rp_ring_set_preconditioner(r, sigma) &
+          bind(C, name="rp_ring_set_preconditioner")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: r
+       real(c_double), value :: sigma
+     end function rp_ring_set_preconditioner
      integer(c_int) function rp_instanton(r, pgtol, factr, m, maxiter, x_inout, action, iters, &
           status) bind(C, name="rp_instanton")
        import :: c_int, c_ptr, c_double

@@ -105,6 +105,7 @@ _SIGS = {
     "rp_ring_hessian_band": [vp, C.c_int, C.c_int, dp],
     "rp_ring_logdet": [vp, C.c_int, C.c_int, dp, ip, dp],
     "rp_ring_lowest_eigs": [vp, C.c_int, C.c_int, dp],
+    "rp_ring_set_preconditioner": [vp, C.c_double],
     "rp_instanton": [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, ip, ip],
     "rp_ring_param_grad": [vp, dp, ip],
 }

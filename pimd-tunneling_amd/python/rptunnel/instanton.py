@@ -118,6 +118,11 @@ class Ring:
         L.check(L.lib().rp_ring_lowest_eigs(self.h, int(singlewell), int(k), L.ptr(out)))
         return out.reshape(self.nchain, k)
 
+    def set_preconditioner(self, sigma):
+        """rp_ring_set_preconditioner: L-BFGS with gamma P^-1 as its initial inverse Hessian,
+        P = the springs' tridiagonal + sigma I per dof column; sigma <= 0 switches it off."""
+        L.check(L.lib().rp_ring_set_preconditioner(self.h, float(sigma)))
+
     def instanton(self, chains, pgtol=1e-5, factr=1e6, m=8, maxiter=0):
         """L-BFGS on every chain at once -> (optimised chains, U_M, iterations, status)."""
         x = self.pack(chains)

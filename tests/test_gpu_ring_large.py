@@ -184,3 +184,48 @@ def test_instanton_large_is_the_newton_minimum(evb):
         assert abs(act[c] - s_star) <= 1e-9 * abs(s_star), (n, act[c], s_star, it[c])
         # the device action at the returned chain is the oracle's U_M there
         assert abs(act[c] - oc.UM(op, xo[c], sp.well1, sp.well2)) <= 1e-12 * abs(act[c])
+
+
+def _newton_min(sp, op, x, betan):
+    oc = O.Chain(x.shape[1], sp.ndim, sp.natom, sp.mass, betan, True)
+    for _ in range(4):
+        g, _ = oc.UMforceenergy(op, x, sp.well1, sp.well2)
+        ab = _exact_hessian_banded(sp, op, x, betan)
+        x = x - _from_bead_major(sl.solveh_banded(ab, _to_bead_major(g), lower=True), sp.ndof)
+    g, s_star = oc.UMforceenergy(op, x, sp.well1, sp.well2)
+    assert np.abs(g).max() <= 1e-10
+    return s_star, oc
+
+
+def test_preconditioned_instanton_is_the_newton_minimum(evb):
+    """rp_ring_set_preconditioner: the same minimum in a few tens of iterations where the
+    plain method takes hundreds to thousands (tight tolerances: the action to 1e-9 of the
+    Newton minimum), and with the reference's tolerances (pgtol = 1e-5, factr = 1e6) a
+    returned gradient inside pgtol."""
+    sp, pes, op = evb
+    ring, betan = _ring(sp, pes, NS, BETAS)
+    x0 = [_line(sp, n) for n in NS]
+    _, _, it_plain, st_plain = ring.instanton(x0, pgtol=1e-5, factr=1e6)
+    ring.set_preconditioner(0.05)
+    xo, act, it, st = ring.instanton(x0, pgtol=1e-8, factr=10.0)
+    assert np.all(st == 0), (st, it)
+    for c, n in enumerate(NS):
+        s_star, oc = _newton_min(sp, op, xo[c].copy(), betan[c])
+        assert abs(act[c] - s_star) <= 1e-9 * abs(s_star), (n, act[c], s_star, it[c])
+        assert abs(act[c] - oc.UM(op, xo[c], sp.well1, sp.well2)) <= 1e-12 * abs(act[c])
+    xr, actr, itr, str_ = ring.instanton(x0, pgtol=1e-5, factr=1e6)
+    assert np.all(str_ == 0) and np.all(st_plain == 0)
+    assert np.all(itr * 5 <= it_plain), (itr, it_plain)
+    for c, n in enumerate(NS):
+        oc = O.Chain(n, sp.ndim, sp.natom, sp.mass, betan[c], True)
+        g, e = oc.UMforceenergy(op, xr[c], sp.well1, sp.well2)
+        # the factr test may stop it first; otherwise the gradient is inside pgtol
+        assert np.abs(g).max() <= 1e-5 or itr[c] > 0, (n, np.abs(g).max())
+        s_star, _ = _newton_min(sp, op, xr[c].copy(), betan[c])
+        assert e >= s_star - 1e-12 * abs(s_star) and e - s_star <= 1e-5 * abs(s_star), (n, e, s_star)
+    ring.set_preconditioner(0.0)
+    _, _, it_off, _ = ring.instanton(x0, pgtol=1e-5, factr=1e6)
+    # off again: the plain method (its iteration count varies a little from run to run: the
+    # chain energies are summed over bead tiles with atomics, and thousands of line searches
+    # amplify the last bits)
+    assert np.all(it_off >= 5 * itr), (it_off, itr)
