@@ -632,58 +632,119 @@ int with_logdet_kernel(int D, F&& f) {
   return f(ring_logdet_kernel<13, 256>, 256);
 }
 
-// Lanes per row of the D x D mat-vecs below: a power of two, at most 64 (one wave), with
-// D rows x lanes within the workgroup.
-__device__ __forceinline__ int matvec_lanes(int D) {
-  int q = 64;
-  while (q > 1 && q * D > kRingThreads) q >>= 1;
-  return q;
+// The block solve y = H^-1 v through the stored factors S_i^-1 (the off-diagonal blocks are
+// -c I and bead 0 is coupled to no other bead), in two sweeps whose serial path never waits
+// for global memory:
+//   forward  (m = 0 .. n-1): t_m = S_m^-1 z_m, with z_0 = v_0, z_1 = v_1 and
+//                            z_{m+1} = v_{m+1} + c t_m for m >= 1;
+//   backward (i = n-2 .. 1): y_i = t_i + c S_i^-1 y_{i+1}, with y_{n-1} = t_{n-1}, y_0 = t_0.
+// The running vector (z or y, D values) sits in LDS, double-buffered; what a step reads from
+// global memory (its block, and v_{m+1} or t_i) does not depend on it, so it is loaded P steps
+// ahead into registers, and the per-step barrier orders LDS only, which leaves those loads in
+// flight across it.  Lanes: Q per block row (row r = tid / Q, lane q), each holding the KPL
+// columns k = q + Q j of its row; a row's sum is a DPP butterfly inside its Q lanes.  The LDS
+// vector is zero beyond D, so clamped column loads add nothing.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  return __builtin_amdgcn_update_dpp(0.0, v, CTRL, 0xF, 0xF, false);
 }
 
-// tmp[r] = sum_k A[r][k] u_k for a D x D block A (global, row-major) and u = u0 + c u1
-// (u1 may be null); row r is summed by Q adjacent lanes and reduced with xor shuffles.
-__device__ __forceinline__ void block_matvec(int D, const double* __restrict__ A, const double* u0,
-                                             const double* u1, double c, double* tmp) {
-  const int tid = threadIdx.x, Q = matvec_lanes(D);
-  const int r = tid / Q, q = tid - r * Q;
-  double s = 0.0;
-  if (r < D)
-    for (int k = q; k < D; k += Q) s += A[r * D + k] * (u1 ? u0[k] + c * u1[k] : u0[k]);
-  for (int o = Q >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (r < D && q == 0) tmp[r] = s;
-  __syncthreads();
+// the sum over each aligned group of Q (2, 4 or 8) lanes, in every lane of the group
+template <int Q>
+__device__ __forceinline__ double row_sum(double s) {
+  s += dpp_mov<0xB1>(s);                         // quad_perm [1,0,3,2]: lane ^ 1
+  if constexpr (Q >= 4) s += dpp_mov<0x4E>(s);   // quad_perm [2,3,0,1]: lane ^ 2
+  if constexpr (Q >= 8) s += dpp_mov<0x141>(s);  // row_half_mirror: lane 7 - l of 8
+  return s;
 }
 
-// y = H^-1 v through the stored block factors (block forward / back substitution).
-__device__ void block_solve(int D, int n, double c, const double* __restrict__ sinv,
-                            double* w /* global, n*D: in v, out y */, double* tmp /* LDS D */) {
-  const int tid = threadIdx.x;
-  // forward: w_i += c S_{i-1}^-1 w_{i-1} for coupled (i-1, i), i >= 2
-  for (int i = 2; i < n; ++i) {
-    block_matvec(D, sinv + (size_t)(i - 1) * D * D, w + (size_t)(i - 1) * D, nullptr, 0.0, tmp);
-    for (int r = tid; r < D; r += blockDim.x) w[(size_t)i * D + r] += c * tmp[r];
-    __syncthreads();
+// A raw buffer over [p, p + bytes): its loads take the lane's byte offset in a VGPR and a
+// uniform byte offset in an SGPR, so a step's loads need no 64-bit address per lane.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_buffer(const double* p, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double buffer_ld(__amdgpu_buffer_rsrc_t b, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(b, voff, soff, 0));
+}
+
+// One sweep; returns the LDS buffer that holds its last vector (t_{n-1} after the forward
+// sweep, which is y_{n-1}).  src: v (forward) or the t the forward sweep left in dst
+// (backward).  Byte offsets within a chain fit 31 bits (the host checks n D^2 8 < 2^31).
+template <int Q, int KPL, int P, bool FWD>
+__device__ __forceinline__ int solve_sweep(int D, int n, double c, const double* __restrict__ S,
+                                           const double* src, double* dst, double* zb, int cur) {
+  constexpr int QK = Q * KPL;
+  const int tid = threadIdx.x, r = tid / Q, q = tid - r * Q;
+  const int rr = min(r, D - 1);
+  const bool wr = q == 0 && r < D;
+  const int K = FWD ? n : n - 2, DD8 = D * D * 8, D8 = D * 8;
+  const __amdgpu_buffer_rsrc_t Sb = raw_buffer(S, (int64_t)n * DD8);
+  const __amdgpu_buffer_rsrc_t Eb = raw_buffer(src, (int64_t)n * D8);
+  int ko[KPL];
+#pragma unroll
+  for (int j = 0; j < KPL; ++j) ko[j] = (rr * D + min(q + Q * j, D - 1)) * 8;
+  const int eo = rr * 8;
+  // bead of step k, and the bead of the row value it reads (clamped past the end)
+  auto bead = [&](int k) { return FWD ? min(k, n - 1) : max(n - 2 - k, 0); };
+  auto ebead = [&](int k) { return FWD ? min(k + 1, n - 1) : max(n - 2 - k, 0); };
+  double a[P][KPL], e[P];
+#pragma unroll
+  for (int s = 0; s < P; ++s) {
+    const int so = bead(s) * DD8;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) a[s][j] = buffer_ld(Sb, ko[j], so);
+    e[s] = buffer_ld(Eb, eo, ebead(s) * D8);
   }
-  // back: y_{n-1} = S^-1 w_{n-1}; y_i = S_i^-1 (w_i + c y_{i+1}) when (i, i+1) coupled
-  for (int i = n - 1; i >= 0; --i) {
-    const bool coupled = (i + 1 < n) && (i >= 1);
-    block_matvec(D, sinv + (size_t)i * D * D, w + (size_t)i * D,
-                 coupled ? w + (size_t)(i + 1) * D : nullptr, c, tmp);
-    for (int r = tid; r < D; r += blockDim.x) w[(size_t)i * D + r] = tmp[r];
-    __syncthreads();
+  for (int k0 = 0; k0 < K; k0 += P) {
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      const int k = k0 + s;
+      if (k < K) {  // block-uniform
+        const double* u = zb + cur * QK;
+        double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+        for (int j = 0; j < KPL; j += 2) {
+          t0 += a[s][j] * u[q + Q * j];
+          if (j + 1 < KPL) t1 += a[s][j + 1] * u[q + Q * (j + 1)];
+        }
+        const double t = row_sum<Q>(t0 + t1), ev = e[s];
+        const int so = bead(k + P) * DD8;
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) a[s][j] = buffer_ld(Sb, ko[j], so);
+        e[s] = buffer_ld(Eb, eo, ebead(k + P) * D8);
+        double* nz = zb + (cur ^ 1) * QK;
+        if (FWD) {
+          if (wr) {
+            dst[(int64_t)k * D + r] = t;
+            nz[r] = k + 1 < n ? (k >= 1 ? ev + c * t : ev) : t;
+          }
+        } else {
+          const double y = ev + c * t;
+          if (wr) {
+            dst[(int64_t)(n - 2 - k) * D + r] = y;
+            nz[r] = y;
+          }
+        }
+        cur ^= 1;
+        lds_sync<kRingThreads>();
+      }
+    }
   }
+  return cur;
 }
 
 // Inverse iteration for the eigenvalue of H nearest the factorisation shift sig:
 // v <- (H - sig)^-1 v / |.|, eta = sig + 1 / (v . (H - sig)^-1 v) for the normalised v.
 // work holds [v | y] per chain (2 n D); fresh = 1 starts from a fixed generic vector,
-// otherwise the previous v is continued (Rayleigh-quotient refinement).
+// otherwise the previous v is continued (Rayleigh-quotient refinement).  D <= Q KPL.
+template <int Q, int KPL, int P>
 __global__ void __launch_bounds__(kRingThreads) ring_eta1_kernel(
     int D, const int* __restrict__ nn, const int64_t* __restrict__ boff,
     const double* __restrict__ betan, const double* __restrict__ sinv, double* __restrict__ work,
     const double* __restrict__ shift, int iters, int fresh, double* __restrict__ eta1) {
+  static_assert(Q * (kRingThreads / Q) == kRingThreads && (Q == 2 || Q == 4 || Q == 8), "lanes per row");
   __shared__ double red[kRingThreads];
-  __shared__ double tmp[kMaxD];
+  __shared__ double zb[2 * Q * KPL];
   const int cix = blockIdx.x, n = nn[cix], tid = threadIdx.x;
   const int64_t tot = (int64_t)n * D;
   const double c = 1.0 / (betan[cix] * betan[cix]);
@@ -691,6 +752,7 @@ __global__ void __launch_bounds__(kRingThreads) ring_eta1_kernel(
   double* y = v + tot;
   const double* S = sinv + (size_t)boff[cix] * D * D;
   const double sig = shift ? shift[cix] : 0.0;
+  if (tid < 2 * Q * KPL) zb[tid] = 0.0;
   double nrm = 0.0;
   for (int64_t e = tid; e < tot; e += blockDim.x) {
     const double t = fresh ? 1.0 + 0.37 * (double)((e * 7919) % 101) / 101.0 : v[e];
@@ -702,9 +764,12 @@ __global__ void __launch_bounds__(kRingThreads) ring_eta1_kernel(
   __syncthreads();
   double eta = 0.0, prev = 0.0;
   for (int it = 0; it < iters; ++it) {
-    for (int64_t e = tid; e < tot; e += blockDim.x) y[e] = v[e];
+    if (tid < D) zb[tid] = v[tid];  // z_0 = v_0 in buffer 0
     __syncthreads();
-    block_solve(D, n, c, S, y, tmp);
+    int cur = solve_sweep<Q, KPL, P, true>(D, n, c, S, v, y, zb, 0);
+    __syncthreads();  // the t of every lane, in y, before the backward sweep reads them
+    solve_sweep<Q, KPL, P, false>(D, n, c, S, y, y, zb, cur);
+    __syncthreads();
     double mu = 0.0, yy = 0.0;
     for (int64_t e = tid; e < tot; e += blockDim.x) {
       mu += v[e] * y[e];
@@ -1065,10 +1130,19 @@ int ring_lowest_eigen(rp_ring* r, double* eta) {
   DevBuf<double> d_shift;
   if ((rc = d_shift.alloc(r->nchain))) return rc;
   std::vector<double> sh(r->nchain, 0.0);
-  ring_eta1_kernel<<<r->nchain, kRingThreads, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p,
-                                                         r->d_sinv.p, r->d_work.p, nullptr, 60, 1,
-                                                         r->d_eta.p);
-  RP_HIP(hipGetLastError());
+  if ((int64_t)r->nmax * r->D * r->D * 8 >= ((int64_t)1 << 31))
+    return set_error(RP_ERR_ARG, "ring log-determinant: a chain's block factors exceed 2 GiB");
+  // lanes per block row and prefetch depth of the block solve: 8 x 4 columns, 12 steps ahead
+  // for D <= 32; 4 x 16 columns, 4 steps ahead up to kMaxD (the registers of a deeper ring)
+  auto eta1 = [&](const double* shift, int iters, int fresh) -> int {
+    auto kern = r->D <= 32 ? ring_eta1_kernel<8, 4, 12> : ring_eta1_kernel<4, 16, 4>;
+    kern<<<r->nchain, kRingThreads, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p,
+                                               r->d_sinv.p, r->d_work.p, shift, iters, fresh,
+                                               r->d_eta.p);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  };
+  if ((rc = eta1(nullptr, 60, 1))) return rc;
   for (int round = 0; round < 3; ++round) {
     if ((rc = r->d_eta.download(sh.data(), r->nchain, r->s))) return rc;
     RP_HIP(hipStreamSynchronize(r->s));
@@ -1083,10 +1157,7 @@ int ring_lowest_eigen(rp_ring* r, double* eta) {
           return RP_OK;
         })))
       return rc;
-    ring_eta1_kernel<<<r->nchain, kRingThreads, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p,
-                                                           r->d_sinv.p, r->d_work.p, d_shift.p, 4, 0,
-                                                           r->d_eta.p);
-    RP_HIP(hipGetLastError());
+    if ((rc = eta1(d_shift.p, 4, 0))) return rc;
   }
   rc = r->d_eta.download(eta, r->nchain, r->s);
   if (!rc) RP_HIP(hipStreamSynchronize(r->s));

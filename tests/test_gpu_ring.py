@@ -118,6 +118,35 @@ def test_logdet_at_well_against_eigvalsh():
         assert abs(ld0[k] - O.lndetj0(ev)) <= 1e-9 * abs(O.lndetj0(ev))
 
 
+@pytest.mark.parametrize("D", [5, 54])
+def test_lowest_eigen_and_logdet_harmonic_against_eigvalsh(D):
+    """eta_1 by inverse iteration on the block factors, and lndetJ with and without it
+    (rpi_ser.f90:221-236, 354-365), on rings of D = 5 and 54 dofs (the two lane layouts of
+    the block solve): a harmonic surface, so the Hessian is that of the springs plus a
+    diagonal, and eigvalsh of the dense band matrix is the check.  One soft heavy dof keeps
+    eta_1 apart from eta_2 (inverse iteration cannot tell a near-degenerate pair apart)."""
+    rng = np.random.default_rng(D)
+    kf, mass = rng.uniform(0.1, 0.4, D), rng.uniform(500.0, 1000.0, D)
+    kf[0], mass[0] = 0.02, 3000.0
+    sp = specs.harmonic(k=kf, c=rng.normal(0.0, 0.3, D), mass=mass)
+    pes = Pes(sp)
+    op = oracle_pes(sp)
+    ns, betan = [7, 33, 120], [25.0, 40.0, 12.0]
+    xs = [sp.well1[:, None] + 0.1 * rng.standard_normal((D, n)) for n in ns]
+    ring = Ring(pes, sp.mass, ns, betan, sp.well1, sp.well2)
+    ring.set_x(xs)
+    ld0, nn0, eta0 = ring.logdet(singlewell=False, skip_lowest=False, want_eta=True)
+    ld, nn, eta = ring.logdet(singlewell=False, skip_lowest=True)
+    for k, n in enumerate(ns):
+        oc = O.Chain(n, sp.ndim, sp.natom, sp.mass, betan[k], True)
+        ev = O.detj_eigs(oc.UMhessian(op, xs[k], False))
+        assert nn0[k] == nn[k] == np.sum(ev < 0) == 0
+        assert abs(eta0[k] - ev[0]) <= 1e-9 * abs(ev[0]) and eta[k] == eta0[k]
+        assert abs(ld0[k] - O.lndetj0(ev)) <= 1e-9 * abs(O.lndetj0(ev))
+        assert abs(ld[k] - O.lndetj(ev)) <= 1e-9 * abs(O.lndetj(ev))
+    ring.close()
+
+
 def _linear_path(sp, n):
     k = np.arange(1, n + 1)
     return (np.outer(sp.well1, n - k) + np.outer(sp.well2, k - 1)) / (n - 1)
