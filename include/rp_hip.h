@@ -235,6 +235,15 @@ int rp_ring_get_x(rp_ring *r, double *x);
 int rp_ring_um(rp_ring *r, double *energy, double *grad);
 /* Hessian-vector product of U_M (forward-over-reverse through the adjoint) */
 int rp_ring_hvp(rp_ring *r, const double *v, double *hv);
+/* With no counterpart in the reference (the reference holds its chains in host arrays):
+ * rp_ring_set_x, rp_ring_um and rp_ring_hvp for callers whose chains already sit in the
+ * device memory of the ring's GPU (torch tensors under torch.autograd): x, grad, v and hv
+ * are device pointers in the same layouts, copied device to device; energy stays a host
+ * array.  The caller's writes to x and v must be complete before the call (synchronise the
+ * stream that made them); the outputs are complete on return. */
+int rp_ring_set_x_device(rp_ring *r, const double *x);
+int rp_ring_um_device(rp_ring *r, double *energy, double *grad);
+int rp_ring_hvp_device(rp_ring *r, const double *v, double *hv);
 /* UMhessian (:155-217) of chain `chain`: the reference's answer(ndof+1, n*ndof), LAPACK
  * lower band storage in column-major order (AB(1+i-j, j) at band[j*(ndof+1) + i-j], 0-based),
  * bead blocks from ndof forward-over-reverse sweeps; singlewell reuses bead 1's block

@@ -642,6 +642,8 @@ __global__ void __launch_bounds__(W * 64, (W <= 4 && UPW == 1) ? 2 : 1)
   __shared__ int s_flag;
   constexpr int DC = DT * 16;
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
+  (void)m;  // lane coordinates of the MFMA fragments: unused by the opaque-copy instances
+  (void)g;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (SGPR)
   const int poly = A.first + blockIdx.x;
   const int N = A.N, NPAD = A.NPAD, D = A.D, LD = A.LD;

@@ -1039,28 +1039,58 @@ extern "C" int rp_ring_um(rp_ring* r, double* energy, double* grad) {
 
 static int ring_blocks(rp_ring* r, int singlewell);
 
-extern "C" int rp_ring_hvp(rp_ring* r, const double* v, double* hv) {
-  RP_REQUIRE(r && v && hv, "rp_ring_hvp: bad arguments");
+// d_hv = H d_v at d_x (queued on the ring's stream)
+static int ring_hvp(rp_ring* r) {
   int rc;
-  if ((rc = r->d_v.upload(v, r->tot, r->s))) return rc;
   RingArgs a = ring_args(r);
+  const dim3 grid((unsigned)((r->nmax + kRingTile - 1) / kRingTile), (unsigned)r->nchain);
   if (r->pes->kind == PES_HOST) {  // the plugin's Hessian blocks, then the block product
     if ((rc = ring_blocks(r, 0))) return rc;
-    const dim3 grid((unsigned)((r->nmax + kRingTile - 1) / kRingTile), (unsigned)r->nchain);
     ring_hvp_blocks_kernel<<<grid, kRingTile, 0, r->s>>>(a, r->d_blocks.p, r->d_v.p, r->d_hv.p);
     RP_HIP(hipGetLastError());
-    if ((rc = r->d_hv.download(hv, r->tot, r->s))) return rc;
-    RP_HIP(hipStreamSynchronize(r->s));
     return RP_OK;
   }
-  rc = dispatch_pes(r->pes, [&]<int K, int M>() -> int {
-    const dim3 grid((unsigned)((r->nmax + kRingTile - 1) / kRingTile), (unsigned)r->nchain);
+  return dispatch_pes(r->pes, [&]<int K, int M>() -> int {
     ring_hvp_kernel<K, M><<<grid, kRingTile, 0, r->s>>>(a, r->d_x.p, r->d_v.p, r->d_hv.p);
     RP_HIP(hipGetLastError());
     return RP_OK;
   });
-  if (rc) return rc;
+}
+
+extern "C" int rp_ring_hvp(rp_ring* r, const double* v, double* hv) {
+  RP_REQUIRE(r && v && hv, "rp_ring_hvp: bad arguments");
+  int rc;
+  if ((rc = r->d_v.upload(v, r->tot, r->s))) return rc;
+  if ((rc = ring_hvp(r))) return rc;
   if ((rc = r->d_hv.download(hv, r->tot, r->s))) return rc;
+  RP_HIP(hipStreamSynchronize(r->s));
+  return RP_OK;
+}
+
+// Device-memory twins of rp_ring_set_x / rp_ring_um / rp_ring_hvp: device-to-device copies
+// on the ring's stream, complete on return.
+extern "C" int rp_ring_set_x_device(rp_ring* r, const double* x) {
+  RP_REQUIRE(r && x, "rp_ring_set_x_device: bad arguments");
+  RP_HIP(hipMemcpyAsync(r->d_x.p, x, sizeof(double) * r->tot, hipMemcpyDeviceToDevice, r->s));
+  RP_HIP(hipStreamSynchronize(r->s));
+  return RP_OK;
+}
+
+extern "C" int rp_ring_um_device(rp_ring* r, double* energy, double* grad) {
+  RP_REQUIRE(r && energy, "rp_ring_um_device: bad arguments");
+  int rc = ring_eval(r, grad != nullptr);
+  if (rc) return rc;
+  if (grad)
+    RP_HIP(hipMemcpyAsync(grad, r->d_g.p, sizeof(double) * r->tot, hipMemcpyDeviceToDevice, r->s));
+  return ring_download(r, energy, nullptr);
+}
+
+extern "C" int rp_ring_hvp_device(rp_ring* r, const double* v, double* hv) {
+  RP_REQUIRE(r && v && hv, "rp_ring_hvp_device: bad arguments");
+  int rc;
+  RP_HIP(hipMemcpyAsync(r->d_v.p, v, sizeof(double) * r->tot, hipMemcpyDeviceToDevice, r->s));
+  if ((rc = ring_hvp(r))) return rc;
+  RP_HIP(hipMemcpyAsync(hv, r->d_hv.p, sizeof(double) * r->tot, hipMemcpyDeviceToDevice, r->s));
   RP_HIP(hipStreamSynchronize(r->s));
   return RP_OK;
 }

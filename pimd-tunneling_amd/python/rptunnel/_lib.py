@@ -102,6 +102,9 @@ _SIGS = {
     "rp_ring_get_x": [vp, dp],
     "rp_ring_um": [vp, dp, dp],
     "rp_ring_hvp": [vp, dp, dp],
+    "rp_ring_set_x_device": [vp, vp],
+    "rp_ring_um_device": [vp, dp, vp],
+    "rp_ring_hvp_device": [vp, vp, vp],
     "rp_ring_hessian_band": [vp, C.c_int, C.c_int, dp],
     "rp_ring_logdet": [vp, C.c_int, C.c_int, dp, ip, dp],
     "rp_ring_lowest_eigs": [vp, C.c_int, C.c_int, dp],

@@ -63,6 +63,20 @@ class Ring:
         L.check(L.lib().rp_ring_get_x(self.h, L.ptr(x)))
         return self.unpack(x)
 
+    # -- chains already in device memory: flat float64 buffers of length off[-1] on the
+    # library's GPU, passed by address, complete before the call (rp_ring_*_device)
+    def set_x_device(self, xptr):
+        L.check(L.lib().rp_ring_set_x_device(self.h, C.c_void_p(xptr)))
+
+    def um_device(self, gptr=None):
+        """U_M per chain (host); the adjoint gradient into the device buffer at gptr."""
+        e = np.zeros(self.nchain)
+        L.check(L.lib().rp_ring_um_device(self.h, L.ptr(e), C.c_void_p(gptr) if gptr else None))
+        return e
+
+    def hvp_device(self, vptr, hvptr):
+        L.check(L.lib().rp_ring_hvp_device(self.h, C.c_void_p(vptr), C.c_void_p(hvptr)))
+
     # -- reference routines
     def UMforceenergy(self, chains=None):
         """-> (gradients per chain, U_M per chain)"""

@@ -71,3 +71,31 @@ def test_autograd_against_oracle(name):
         seg = slice(off, off + n * D)
         assert rel(hv[seg].numpy(), float(w[k]) * H @ v[seg].numpy()) <= 1e-11
         off += n * D
+
+
+@pytest.mark.parametrize("name", ["coupled2d", "evb"])
+def test_device_tensors_match_host_tensors(name):
+    """Tensors on the GPU go through the device-memory entry points (no host copies of the
+    chains); every derivative equals the host-tensor path's, and both second-derivative
+    paths (d/dx and d/dw of the weighted gradient) run on the device."""
+    ns, betan = [7, 4, 12], [11.0, 6.0, 9.0]
+    sp, pes, ring, xs, x = _setup(name, ns, betan, seed=12)
+    dev = torch.device("cuda", 0)
+    xd = x.detach().to(dev).requires_grad_(True)
+    w = torch.tensor([0.7, -1.3, 0.4], dtype=torch.float64, requires_grad=True)
+    wd = w.detach().to(dev).requires_grad_(True)
+    v = torch.tensor(np.random.default_rng(5).standard_normal(x.numel()), dtype=torch.float64)
+    out = []
+    for xx, ww, vv in ((x, w, v), (xd, wd, v.to(dev))):
+        e = ring_action(xx, ring)
+        (gx,) = torch.autograd.grad((e * ww).sum(), xx, create_graph=True)
+        hv, dw = torch.autograd.grad(gx, (xx, ww), grad_outputs=vv)
+        out.append([t.detach().cpu().numpy() for t in (e, gx, hv, dw)])
+        if xx.is_cuda:
+            assert all(t.device == dev for t in (e, gx, hv, dw))
+    for h, d in zip(*out):
+        assert rel(d, h) <= 1e-14
+    # d/dw_c of sum_c w_c grad U_c . v is grad U_c . v
+    g, _ = ring.UMforceenergy(xs)
+    want = [float(np.dot(g[c].reshape(-1), v.numpy()[ring.off[c]:ring.off[c + 1]])) for c in range(3)]
+    assert rel(out[1][3], np.array(want)) <= 1e-13
