@@ -13,7 +13,7 @@ int ring_nchain(const rp_ring* r);
 int ring_D(const rp_ring* r);
 const std::vector<int>& ring_n(const rp_ring* r);
 const std::vector<int64_t>& ring_off(const rp_ring* r);
-int ring_lowest_eigen(rp_ring* r, double* eta);
+int ring_lowest_eigen(rp_ring* r, const std::vector<int>& ng, double* eta);
 // device views used by the instanton search: the evaluation point x, its adjoint gradient
 // g and energies e (written by ring_eval), chain offsets and bead counts, the stream
 struct RingBufs {

@@ -495,9 +495,10 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
     const double* __restrict__ blocks, const double* __restrict__ shift, double* __restrict__ sinv,
     double* __restrict__ logdet, int* __restrict__ nneg, const int* __restrict__ chain_of) {
   // every LDS element t + T q exists, and so does every r D + k <= t + T q + D - 1 read for
-  // it; the elements beyond D*D hold values nobody reads back, which keeps every load and
-  // store unconditional (no branch, so the loads of the next block stay in flight)
-  constexpr int kTile = T * P + kMaxD > kMaxD * (kMaxD + 1) ? T * P + kMaxD : kMaxD * (kMaxD + 1);
+  // it (and k D + cc < D^2 <= T P); the elements beyond D*D hold values nobody reads back,
+  // which keeps every load and store unconditional (no branch, so the loads of the next
+  // block stay in flight).  Sized by T P, so several workgroups share a CU's LDS.
+  constexpr int kTile = T * P + kMaxD;
   __shared__ double Sb[2][kTile];
   // one task per workgroup: chain c (task t of a chain_of map, else chain t) factored at
   // the shift of task t; outputs per task
@@ -736,16 +737,20 @@ __device__ __forceinline__ int solve_sweep(int D, int n, double c, const double*
 // Inverse iteration for the eigenvalue of H nearest the factorisation shift sig:
 // v <- (H - sig)^-1 v / |.|, eta = sig + 1 / (v . (H - sig)^-1 v) for the normalised v.
 // work holds [v | y] per chain (2 n D); fresh = 1 starts from a fixed generic vector,
-// otherwise the previous v is continued (Rayleigh-quotient refinement).  D <= Q KPL.
+// otherwise the previous v is continued.  Stops after iters iterations, or earlier once eta
+// changes by at most tol relative; chains with act[c] == 0 are left as they are.
+// D <= Q KPL.
 template <int Q, int KPL, int P>
 __global__ void __launch_bounds__(kRingThreads) ring_eta1_kernel(
     int D, const int* __restrict__ nn, const int64_t* __restrict__ boff,
     const double* __restrict__ betan, const double* __restrict__ sinv, double* __restrict__ work,
-    const double* __restrict__ shift, int iters, int fresh, double* __restrict__ eta1) {
+    const double* __restrict__ shift, int iters, int fresh, double tol,
+    const int* __restrict__ act, double* __restrict__ eta1) {
   static_assert(Q * (kRingThreads / Q) == kRingThreads && (Q == 2 || Q == 4 || Q == 8), "lanes per row");
   __shared__ double red[kRingThreads];
   __shared__ double zb[2 * Q * KPL];
   const int cix = blockIdx.x, n = nn[cix], tid = threadIdx.x;
+  if (act && !act[cix]) return;  // block-uniform
   const int64_t tot = (int64_t)n * D;
   const double c = 1.0 / (betan[cix] * betan[cix]);
   double* v = work + 2 * boff[cix] * D;
@@ -780,7 +785,7 @@ __global__ void __launch_bounds__(kRingThreads) ring_eta1_kernel(
     for (int64_t e = tid; e < tot; e += blockDim.x) v[e] = y[e] / yy;
     __syncthreads();
     eta = sig + 1.0 / mu;
-    if (it > 4 && fabs(eta - prev) <= 1e-15 * fabs(eta)) break;  // block-uniform
+    if (it > 2 && fabs(eta - prev) <= tol * fabs(eta)) break;  // block-uniform
     prev = eta;
   }
   if (tid == 0) eta1[cix] = eta;
@@ -1150,49 +1155,161 @@ extern "C" int rp_ring_hessian_band(rp_ring* r, int chain, int singlewell, doubl
 }
 
 namespace rp {
-// Eigenvalue of every chain's Hessian nearest zero: inverse iteration on the unshifted
-// factors already in d_sinv, then Rayleigh-quotient refinement (refactor H - eta I,
-// a few shifted inverse iterations), which converges even when eta_2 / eta_1 is near 1.
-// Leaves d_sinv holding shifted factors.
-int ring_lowest_eigen(rp_ring* r, double* eta) {
+// Factor every task t (chain chain_of[t], or chain t when chain_of is null) at shift[t]:
+// log|det|, the number of negative pivots (eigenvalues below the shift) and, when keep, the
+// S_i^-1 of the chain for inverse iteration.
+static int ring_factor(rp_ring* r, int ntask, const double* shift, const int* chain_of,
+                       bool keep, double* ld, int* nneg) {
+  return with_logdet_kernel(r->D, [&](auto kern, int nt) -> int {
+    kern<<<ntask, nt, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p, r->d_massd.p,
+                                r->d_blocks.p, shift, keep ? r->d_sinv.p : nullptr, ld, nneg,
+                                chain_of);
+    RP_HIP(hipGetLastError());
+    return RP_OK;
+  });
+}
+
+// eta[c] = the lowest eigenvalue of chain c's Hessian when it has none below zero (ng[c] == 0),
+// else the eigenvalue nearest zero (which the caller keeps only when it is the one negative
+// eigenvalue).  d_sinv holds the unshifted factors on entry; it is left holding shifted ones.
+//
+// The bottom of an instanton's spectrum can be a cluster (the springs' modes lie a few per
+// cent apart at large N), where inverse iteration from zero converges slowly and a Rayleigh
+// quotient shift can land on any member.  So for ng[c] == 0 the search keeps a bracket on
+// Sylvester's inertia: lo with no eigenvalue below it (0 to start), hi with at least one (the
+// Rayleigh quotient of a few unshifted inverse iterations, an upper bound on eta_1), and h1,
+// the largest shift seen with exactly one below it (so eta_2 >= h1).  Rounds of multisection
+// (three shifts per chain in one launch) narrow it until inverse iteration shifted to lo,
+// which converges to the eigenvalue nearest lo, i.e. eta_1, contracts by
+// (hi - lo) / (h1 - lo) <= 1/4 per iteration; that iteration then gives eta_1 to the last
+// bits.  A chain whose bracket closes first takes its midpoint.
+int ring_lowest_eigen(rp_ring* r, const std::vector<int>& ng, double* eta) {
+  const int C = r->nchain;
   int rc;
-  if (r->d_work.n < 2 * (size_t)r->tot && (rc = r->d_work.alloc(2 * (size_t)r->tot))) return rc;
-  DevBuf<double> d_shift;
-  if ((rc = d_shift.alloc(r->nchain))) return rc;
-  std::vector<double> sh(r->nchain, 0.0);
   if ((int64_t)r->nmax * r->D * r->D * 8 >= ((int64_t)1 << 31))
     return set_error(RP_ERR_ARG, "ring log-determinant: a chain's block factors exceed 2 GiB");
+  if (r->d_work.n < 2 * (size_t)r->tot && (rc = r->d_work.alloc(2 * (size_t)r->tot))) return rc;
+  constexpr int M = 7;  // shifts per chain and multisection round
+  DevBuf<double> d_shift, d_tld;
+  DevBuf<int> d_act, d_map, d_cnt;
+  auto cleanup = [&]() { d_shift.free(); d_tld.free(); d_act.free(); d_map.free(); d_cnt.free(); };
+  if ((rc = d_shift.alloc(M * (size_t)C)) || (rc = d_tld.alloc(M * (size_t)C)) ||
+      (rc = d_act.alloc(C)) || (rc = d_map.alloc(M * (size_t)C)) || (rc = d_cnt.alloc(M * (size_t)C))) {
+    cleanup();
+    return rc;
+  }
   // lanes per block row and prefetch depth of the block solve: 8 x 4 columns, 12 steps ahead
   // for D <= 32; 4 x 16 columns, 4 steps ahead up to kMaxD (the registers of a deeper ring)
-  auto eta1 = [&](const double* shift, int iters, int fresh) -> int {
+  auto eta1 = [&](const double* shift, int iters, int fresh, double tol, const int* act) -> int {
     auto kern = r->D <= 32 ? ring_eta1_kernel<8, 4, 12> : ring_eta1_kernel<4, 16, 4>;
-    kern<<<r->nchain, kRingThreads, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p,
-                                               r->d_sinv.p, r->d_work.p, shift, iters, fresh,
-                                               r->d_eta.p);
+    kern<<<C, kRingThreads, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p, r->d_sinv.p,
+                                       r->d_work.p, shift, iters, fresh, tol, act, r->d_eta.p);
     RP_HIP(hipGetLastError());
     return RP_OK;
   };
-  if ((rc = eta1(nullptr, 60, 1))) return rc;
-  for (int round = 0; round < 3; ++round) {
-    if ((rc = r->d_eta.download(sh.data(), r->nchain, r->s))) return rc;
-    RP_HIP(hipStreamSynchronize(r->s));
-    // shift just off the estimate so the shifted matrix stays factorable
-    for (double& v : sh) v = v * (1.0 - 1e-9);
-    if ((rc = d_shift.upload(sh.data(), r->nchain, r->s))) return rc;
-    if ((rc = with_logdet_kernel(r->D, [&](auto kern, int nt) -> int {
-          kern<<<r->nchain, nt, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p, r->d_massd.p,
-                                          r->d_blocks.p, d_shift.p, r->d_sinv.p, r->d_ld.p,
-                                          r->d_nneg.p, nullptr);
-          RP_HIP(hipGetLastError());
-          return RP_OK;
-        })))
-      return rc;
-    if ((rc = eta1(d_shift.p, 4, 0))) return rc;
+  auto fetch_eta = [&](std::vector<double>& e) -> int {
+    int q = r->d_eta.download(e.data(), C, r->s);
+    if (!q && hipStreamSynchronize(r->s) != hipSuccess) q = set_error(RP_ERR_HIP, "inverse iteration");
+    return q;
+  };
+  std::vector<double> mu(C), sh(C, 0.0);
+  std::vector<int> act(C);
+  // inverse iteration on the unshifted factors, from a fixed generic start
+  if ((rc = eta1(nullptr, 12, 1, 0.0, nullptr)) || (rc = fetch_eta(mu))) return cleanup(), rc;
+  // chains with negative eigenvalues: Rayleigh-quotient refinement toward the one nearest zero
+  bool any = false;
+  for (int c = 0; c < C; ++c) any |= (act[c] = ng[c] != 0);
+  for (int round = 0; any && round < 3; ++round) {
+    std::vector<int> map;
+    for (int c = 0; c < C; ++c) {
+      sh[c] = mu[c] * (1.0 - 1e-9);  // just off the estimate, so H - sh I stays factorable
+      if (act[c]) map.push_back(c);
+    }
+    std::vector<double> tsh;
+    for (int c : map) tsh.push_back(sh[c]);
+    if ((rc = d_map.upload(map.data(), map.size(), r->s)) ||
+        (rc = d_tld.upload(tsh.data(), tsh.size(), r->s)) ||
+        (rc = ring_factor(r, (int)map.size(), d_tld.p, d_map.p, true, d_shift.p + C, d_cnt.p)) ||
+        (rc = d_shift.upload(sh.data(), C, r->s)) || (rc = d_act.upload(act.data(), C, r->s)) ||
+        (rc = eta1(d_shift.p, 4, 0, 1e-15, d_act.p)) || (rc = fetch_eta(mu)))
+      return cleanup(), rc;
   }
-  rc = r->d_eta.download(eta, r->nchain, r->s);
-  if (!rc) RP_HIP(hipStreamSynchronize(r->s));
-  d_shift.free();
-  return rc;
+  // chains with none: the lowest eigenvalue on an inertia bracket
+  std::vector<double> lo(C, 0.0), hi(C), h1(C, -1.0);
+  std::vector<char> open(C, 0);
+  for (int c = 0; c < C; ++c) {
+    eta[c] = mu[c];
+    if (ng[c] != 0) continue;
+    if (!(mu[c] > 0.0))
+      return cleanup(), set_error(RP_ERR_CONVERGE, "chain %d: inverse iteration gave eta = %g with no negative eigenvalue", c, mu[c]);
+    hi[c] = mu[c] * (1.0 + 1e-13);
+    open[c] = 1;
+  }
+  auto needs_more = [&](int c) {
+    if (hi[c] - lo[c] <= 4e-16 * hi[c]) return false;  // closed to a few ulps
+    return !(h1[c] > lo[c] && hi[c] - lo[c] <= 0.25 * (h1[c] - lo[c]));
+  };
+  for (int round = 0; round < 60; ++round) {
+    std::vector<int> map;
+    std::vector<double> tsh;
+    for (int c = 0; c < C; ++c) {
+      if (!open[c] || !needs_more(c)) continue;
+      for (int k = 1; k <= M; ++k) {
+        map.push_back(c);
+        // the first round's shifts crowd toward hi (eta_1 is usually within a few per cent
+        // of the Rayleigh quotient), later rounds split the bracket evenly
+        const double f = round == 0 ? 1.0 - std::ldexp(1.0, -k) : (double)k / (M + 1);
+        tsh.push_back(lo[c] + (hi[c] - lo[c]) * f);
+      }
+    }
+    if (map.empty()) break;
+    std::vector<int> cnt(map.size());
+    if ((rc = d_map.upload(map.data(), map.size(), r->s)) ||
+        (rc = d_tld.upload(tsh.data(), tsh.size(), r->s)) ||
+        (rc = ring_factor(r, (int)map.size(), d_tld.p, d_map.p, false, d_shift.p, d_cnt.p)) ||
+        (rc = d_cnt.download(cnt.data(), cnt.size(), r->s)))
+      return cleanup(), rc;
+    if (hipStreamSynchronize(r->s) != hipSuccess) return cleanup(), set_error(RP_ERR_HIP, "inertia count");
+    // the counts rise with the shift: the bracket closes onto the first rise
+    for (size_t t = 0; t < map.size(); ++t) {
+      const int c = map[t];
+      const double x = tsh[t];
+      if (cnt[t] == 0) lo[c] = std::max(lo[c], x);
+      else hi[c] = std::min(hi[c], x);
+      if (cnt[t] == 1) h1[c] = std::max(h1[c], x);
+    }
+  }
+  // inverse iteration shifted to lo: every eigenvalue lies above it, so it converges to eta_1
+  std::vector<int> map;
+  std::vector<double> tsh;
+  for (int c = 0; c < C; ++c) {
+    act[c] = 0;
+    if (!open[c]) continue;
+    if (hi[c] - lo[c] <= 4e-16 * hi[c]) {
+      eta[c] = 0.5 * (lo[c] + hi[c]);
+      continue;
+    }
+    act[c] = 1;
+    sh[c] = lo[c];
+    map.push_back(c);
+    tsh.push_back(lo[c]);
+  }
+  if (!map.empty()) {
+    std::vector<double> e(C);
+    if ((rc = d_map.upload(map.data(), map.size(), r->s)) ||
+        (rc = d_tld.upload(tsh.data(), tsh.size(), r->s)) ||
+        (rc = ring_factor(r, (int)map.size(), d_tld.p, d_map.p, true, d_shift.p + C, d_cnt.p)) ||
+        (rc = d_shift.upload(sh.data(), C, r->s)) || (rc = d_act.upload(act.data(), C, r->s)) ||
+        (rc = eta1(d_shift.p, 30, 0, 1e-14, d_act.p)) || (rc = fetch_eta(e)))
+      return cleanup(), rc;
+    for (int c : map) {
+      if (!(e[c] > lo[c] && e[c] <= hi[c] * (1.0 + 1e-12)))
+        return cleanup(), set_error(RP_ERR_CONVERGE, "chain %d: eta_1 = %.17g left its inertia bracket [%.17g, %.17g]", c, e[c], lo[c], hi[c]);
+      eta[c] = e[c];
+    }
+  }
+  cleanup();
+  return RP_OK;
 }
 }  // namespace rp
 
@@ -1341,7 +1458,7 @@ extern "C" int rp_ring_logdet(rp_ring* r, int singlewell, int skip_lowest, doubl
   RP_HIP(hipStreamSynchronize(r->s));
   const bool want_near = skip_lowest || eta1;
   if (want_near) {
-    if ((rc = ring_lowest_eigen(r, et.data()))) return rc;
+    if ((rc = ring_lowest_eigen(r, ng, et.data()))) return rc;
   }
   // log|det| = sum ln|eta| over every eigenvalue, so the sum over the positive ones is log|det|
   // less ln|eta| over the negative ones.  With one negative eigenvalue that is the eigenvalue

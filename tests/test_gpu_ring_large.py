@@ -229,3 +229,36 @@ def test_preconditioned_instanton_is_the_newton_minimum(evb):
     # chain energies are summed over bead tiles with atomics, and thousands of line searches
     # amplify the last bits)
     assert np.all(it_off >= 5 * itr), (it_off, itr)
+
+
+def test_lowest_eigenvalue_in_a_clustered_spectrum(evb):
+    """lndetJ at instantons of the batched-instanton workload whose lowest eigenvalues lie a
+    few per cent apart (three of its chains: N = 668, 499, 581 at their beta).  There inverse
+    iteration from zero barely separates eta_1 from eta_2 and a Rayleigh-quotient shift can
+    land on either, so eta_1 is held by an inertia bracket.  References: LAPACK's dsbevx
+    (scipy.linalg.eig_banded, lowest eigenvalues only) on the band and the banded Cholesky's
+    log-determinant; lndetJ to 1e-9, eta_1 (about 1e-6 of a band whose norm is about 1, so
+    both factorisations fix it only to some 1e-15 absolute) to 1e-8, which still tells it
+    from eta_2 (1.2e-4 above it at N = 668)."""
+    sp, pes, op = evb
+    ns = [668, 499, 581]
+    betas = [1657.8751797669652, 1435.731658886481, 3046.54219889159]
+    ring, betan = _ring(sp, pes, ns, betas)
+    ring.set_preconditioner(0.05)
+    xo, act, it, st = ring.instanton([_line(sp, n) for n in ns], pgtol=1e-5, factr=1e6)
+    assert np.all(st == 0), (st, it)
+    ring.set_x(xo)
+    ld, nn, eta = ring.logdet(singlewell=False, skip_lowest=True)
+    gaps = []
+    for c, n in enumerate(ns):
+        band = ring.UMhessian(c, singlewell=False)
+        ev = sl.eig_banded(band, lower=True, eigvals_only=True, select="i", select_range=(0, 2))
+        L = sl.cholesky_banded(band, lower=True)
+        logdet = 2.0 * float(np.sum(np.log(L[0])))
+        assert nn[c] == 0 and ev[0] > 0.0
+        assert abs(eta[c] - ev[0]) <= 1e-8 * ev[0], (n, eta[c], ev)
+        want = logdet - float(np.log(ev[0]))  # rpi_ser.f90:355-365: eta_1 skipped
+        assert abs(ld[c] - want) <= 1e-9 * abs(want), (n, ld[c], want)
+        gaps.append((ev[1] - ev[0]) / ev[0])
+    assert min(gaps) < 0.05, gaps  # the case this test is for
+    ring.close()
