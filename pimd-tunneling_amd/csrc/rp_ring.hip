@@ -462,8 +462,8 @@ constexpr int kMaxD = 56;  // ndof bound of the log-determinant workspace (LDS)
 // One wave per chain.  Lane l owns the elements e = l + 64 q (q < P) of every D x D block, so
 // the Hessian block of the next bead is loaded into registers while the current one is
 // eliminated, and the Schur update uses the previous inverse where it sits in LDS.  The
-// Gauss-Jordan inverse runs without pivoting, double-buffered in LDS (one barrier per
-// elimination): the pivot of elimination k is the k-th pivot of the matrix's LDL^T
+// Gauss-Jordan inverse runs without pivoting, double-buffered in LDS (one barrier per pair
+// of eliminations): the pivot of elimination k is the k-th pivot of the matrix's LDL^T
 // factorisation (a ratio of leading principal minors), so their logs sum to log|det| and
 // their signs give the inertia (Sylvester).
 // Barrier of ring_logdet_kernel: LDS ordering only, so the global loads of the next block
@@ -484,8 +484,8 @@ __device__ __forceinline__ void lds_sync() {
 // T threads per chain.  Thread t owns the elements e = t + T q (q < P) of every D x D block,
 // so the Hessian block of the next bead is loaded into registers while the current one is
 // eliminated, and the Schur update uses the previous inverse where it sits in LDS.  The
-// Gauss-Jordan inverse runs without pivoting, double-buffered in LDS (one barrier per
-// elimination): the pivot of elimination k is the k-th pivot of the matrix's LDL^T
+// Gauss-Jordan inverse runs without pivoting, double-buffered in LDS (one barrier per pair
+// of eliminations): the pivot of elimination k is the k-th pivot of the matrix's LDL^T
 // factorisation (a ratio of leading principal minors), so their logs sum to log|det| and
 // their signs give the inertia (Sylvester).
 template <int P, int T>
@@ -541,19 +541,45 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
 #pragma unroll
     for (int q = 0; q < P; ++q) nxt[q] = nb[eg[q]];
     lds_sync<T>();
+    // eliminations two pivots at a time (a 2 x 2 pivot block, inverted explicitly, is two
+    // successive Gauss-Jordan steps), so a bead takes ceil(D/2) barriers; the pivots of the
+    // pair are a and det/a, the same two LDL^T pivots as one at a time
     double mypiv = 1.0;
-    for (int k = 0; k < D; ++k) {
+    for (int k = 0; k < D; k += 2) {
       const double* A = Sb[cur];
       double* B = Sb[cur ^ 1];
-      const double p = A[k * D + k], ip = 1.0 / p;
-      mypiv = tid == k ? p : mypiv;
+      if (k + 1 < D) {
+        const int l = k + 1;
+        const double a = A[k * D + k], b = A[k * D + l], cq = A[l * D + k], d = A[l * D + l];
+        const double det = fma(a, d, -b * cq), id = 1.0 / det;
+        const double p00 = d * id, p01 = -b * id, p10 = -cq * id, p11 = a * id;  // block^-1
+        mypiv = tid == k ? a : tid == l ? det / a : mypiv;
 #pragma unroll
-      for (int q = 0; q < P; ++q) {
-        const int e = tid + T * q, r = er[q], cc = ec[q];
-        const double mrk = A[r * D + k], mkc = A[k * D + cc], a = A[e];
-        const double vr = cc == k ? ip : mkc * ip;
-        const double vo = cc == k ? -mrk * ip : a - mrk * mkc * ip;
-        B[e] = r == k ? vr : vo;
+        for (int q = 0; q < P; ++q) {
+          const int e = tid + T * q, r = er[q], cc = ec[q];
+          const double u0 = A[r * D + k], u1 = A[r * D + l], w0 = A[k * D + cc], w1 = A[l * D + cc];
+          const double ae = A[e];
+          const double pw0 = p00 * w0 + p01 * w1, pw1 = p10 * w0 + p11 * w1;  // block^-1 A[K][cc]
+          const double up0 = u0 * p00 + u1 * p10, up1 = u0 * p01 + u1 * p11;  // A[r][K] block^-1
+          const bool rk = r == k || r == l, ck = cc == k || cc == l;
+          double v = ae - (u0 * pw0 + u1 * pw1);
+          v = rk ? (r == k ? pw0 : pw1) : v;
+          v = ck ? (rk ? (r == k ? (cc == k ? p00 : p01) : (cc == k ? p10 : p11))
+                       : -(cc == k ? up0 : up1))
+                 : v;
+          B[e] = v;
+        }
+      } else {
+        const double p = A[k * D + k], ip = 1.0 / p;
+        mypiv = tid == k ? p : mypiv;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          const int e = tid + T * q, r = er[q], cc = ec[q];
+          const double mrk = A[r * D + k], mkc = A[k * D + cc], a = A[e];
+          const double vr = cc == k ? ip : mkc * ip;
+          const double vo = cc == k ? -mrk * ip : a - mrk * mkc * ip;
+          B[e] = r == k ? vr : vo;
+        }
       }
       cur ^= 1;
       lds_sync<T>();
