@@ -27,6 +27,21 @@ struct RingArgs {
   const double* massd;     // per dof
 };
 
+// A barrier that orders LDS only, so global loads and stores stay in flight across it (a
+// one-wave workgroup needs no hardware barrier at all: its LDS operations complete in issue
+// order).
+template <int T>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (T == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  }
+}
+
 __device__ __forceinline__ double block_sum(double v, double* red) {
   const int tid = threadIdx.x;
   red[tid] = v;
@@ -95,13 +110,18 @@ __global__ void __launch_bounds__(kRingTile) ring_um_kernel(RingArgs A, const do
       if (grad) grad[A.off[c] + (size_t)d * n + i] = sg + gs[d];
     }
   }
-  red[threadIdx.x] = e;
-  __syncthreads();
-  for (int st = kRingTile / 2; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
+  // wave sums, then the waves' through LDS behind an LDS-only barrier (no wait for the
+  // gradient stores)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  lds_sync<kRingTile>();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kRingTile / 64; ++w) t += red[w];
+    atomicAdd(&energy[c], t);
   }
-  if (threadIdx.x == 0) atomicAdd(&energy[c], red[0]);
 }
 
 // U_M and its adjoint gradient for the EVB on the matrix cores.  A wave takes a tile of 16
@@ -114,11 +134,15 @@ __global__ void __launch_bounds__(kRingTile) ring_um_kernel(RingArgs A, const do
 typedef double rd4 __attribute__((ext_vector_type(4)));
 constexpr int kUmWaves = 4;
 
+// Persistent: the grid is a few workgroups per CU, each staging the EVB block once and then
+// taking bead tiles from the task list (tasks[2t] = chain, tasks[2t + 1] = first bead).
 template <int M>
 __global__ void __launch_bounds__(kUmWaves * 64) ring_um_evb_kernel(RingArgs A,
                                                                     const double* __restrict__ x,
                                                                     double* __restrict__ grad,
-                                                                    double* __restrict__ energy) {
+                                                                    double* __restrict__ energy,
+                                                                    const int* __restrict__ tasks,
+                                                                    int ntask) {
   constexpr int RT = (M + 15) / 16;  // row tiles per well, and dof tiles
   constexpr int DC = 16 * RT;        // padded dofs
   constexpr int LX = DC + 1;         // LDS stride of one bead
@@ -131,137 +155,178 @@ __global__ void __launch_bounds__(kUmWaves * 64) ring_um_evb_kernel(RingArgs A,
   __shared__ double gp[TB * LX];        // the potential gradient of beads b0 .. b0+TB-1
   __shared__ double vpot[TB];
   __shared__ double red[NT];
-  const int c = blockIdx.y;
-  const int n = A.n[c], D = A.D;
-  const int b0 = blockIdx.x * TB;
-  if (b0 >= n) return;  // the same for the whole workgroup
+  __shared__ double sm[3 * M];  // per dof: mass, end bead a, end bead b
+  const int D = A.D;
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const double* __restrict__ xc = x + A.off[c];
   {
-    // distinct global sources and LDS targets: the loads of several iterations are in flight
     const double* __restrict__ dp = A.pes.dp;
-#pragma unroll 4
-    for (int e = tid; e < evb_dev_size<M>(); e += NT) pl[e] = dp[e];
-#pragma unroll 4
-    for (int e = tid; e < (TB + 2) * DC; e += NT) {
-      const int d = e / (TB + 2), bl = e - d * (TB + 2);  // consecutive threads, consecutive beads
-      const int i = b0 - 1 + bl;
-      xs[bl * LX + d] = (d < D && i >= 0 && i < n) ? xc[(size_t)d * n + i] : 0.0;
+    constexpr int NPL = (evb_dev_size<M>() + NT - 1) / NT;
+    double vpl[NPL];
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      const int e = tid + q * NT;
+      vpl[q] = e < evb_dev_size<M>() ? dp[e] : 0.0;
     }
+#pragma unroll
+    for (int q = 0; q < NPL; ++q)
+      if (tid + q * NT < evb_dev_size<M>()) pl[tid + q * NT] = vpl[q];
   }
-  __syncthreads();
-  {
-    const double kappa = pl[0];
-    const double* WA = pl + 1;
-    const double* cA = WA + M * M;
-    const double* WB = cA + M;
-    const double* cB = WB + M * M;
-    const double* lam = cB + M;
-    const double* cub = lam + M;
-    const int tb = wave * 16;
-    rd4 ay[2 * RT];
+  for (int task = blockIdx.x; task < ntask; task += gridDim.x) {
+    const int c = tasks[2 * task], b0 = tasks[2 * task + 1];
+    const int n = A.n[c];
+    const double* __restrict__ xc = x + A.off[c];
+    {
+      // every staging load of a thread is issued before the first one is used (compile-time
+      // trip counts, registers in between): one memory latency for the whole staging, and it
+      // overlaps the previous tile's last barrier
+      const double* __restrict__ abc = A.ab + (size_t)c * 2 * D;
+      constexpr int NXS = ((TB + 2) * DC + NT - 1) / NT;
+      double vxs[NXS], vsm = 0.0;
 #pragma unroll
-    for (int r = 0; r < 2 * RT; ++r) ay[r] = rd4{0.0, 0.0, 0.0, 0.0};
+      for (int q = 0; q < NXS; ++q) {
+        const int e = tid + q * NT;
+        // consecutive threads, consecutive beads
+        const int d = e / (TB + 2), bl = e - d * (TB + 2);
+        const int i = b0 - 1 + bl;
+        vxs[q] = (d < D && i >= 0 && i < n) ? xc[(size_t)d * n + i] : 0.0;
+      }
+      if (tid < 3 * D) vsm = tid < D ? A.massd[tid] : abc[tid - D];
+      lds_sync<NT>();  // the previous tile is done with xs, gp, vpot and sm
 #pragma unroll
-    for (int kc = 0; kc < KY; ++kc) {
-      const int j = 4 * kc + g;
-      const double bx = xs[(tb + m + 1) * LX + j];
+      for (int q = 0; q < NXS; ++q) {
+        const int e = tid + q * NT;
+        const int d = e / (TB + 2), bl = e - d * (TB + 2);
+        if (e < (TB + 2) * DC) xs[bl * LX + d] = vxs[q];
+      }
+      if (tid < 3 * D) sm[tid] = vsm;
+    }
+    lds_sync<NT>();
+    {
+      const double kappa = pl[0];
+      const double* WA = pl + 1;
+      const double* cA = WA + M * M;
+      const double* WB = cA + M;
+      const double* cB = WB + M * M;
+      const double* lam = cB + M;
+      const double* cub = lam + M;
+      const int tb = wave * 16;
+      rd4 ay[2 * RT];
 #pragma unroll
-      for (int r = 0; r < 2 * RT; ++r) {
-        const int i = (r % RT) * 16 + m;
+      for (int r = 0; r < 2 * RT; ++r) ay[r] = rd4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kc = 0; kc < KY; ++kc) {
+        const int j = 4 * kc + g;
+        const double bx = xs[(tb + m + 1) * LX + j];
+#pragma unroll
+        for (int r = 0; r < 2 * RT; ++r) {
+          const int i = (r % RT) * 16 + m;
+          const double* Wt = r < RT ? WA : WB;
+          const double w = Wt[min(i, M - 1) * M + min(j, M - 1)];
+          ay[r] = __builtin_amdgcn_mfma_f64_16x16x4f64((i < M && j < M) ? w : 0.0, bx, ay[r], 0,
+                                                       0, 0);
+        }
+      }
+      double va = 0.0, vb = 0.0;
+#pragma unroll
+      for (int r = 0; r < 2 * RT; ++r)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int i = (r % RT) * 16 + g + 4 * jj;
+          const bool ok = i < M;
+          const int ic = min(i, M - 1);
+          const double cw = ok ? (r < RT ? cA[ic] : cB[ic]) : 0.0;
+          const double li = ok ? lam[ic] : 0.0, ci = ok ? cub[ic] : 0.0;
+          const double y = ay[r][jj] - cw, y2 = y * y;
+          const double v = (0.5 * li) * y2 + ci * y2 * y;
+          if (r < RT)
+            va += v;
+          else
+            vb += v;
+          ay[r][jj] = li * y + (3.0 * ci) * y2;
+        }
+      va += __shfl_xor(va, 16);
+      va += __shfl_xor(va, 32);
+      vb += __shfl_xor(vb, 16);
+      vb += __shfl_xor(vb, 32);
+      const double dl = va - vb, R = sqrt(0.25 * dl * dl + kappa * kappa), f = dl / (4.0 * R);
+      const double wa = 0.5 - f, wb = 0.5 + f;
+#pragma unroll
+      for (int r = 0; r < 2 * RT; ++r) ay[r] *= (r < RT ? wa : wb);
+      rd4 ag[RT];
+#pragma unroll
+      for (int d = 0; d < RT; ++d) ag[d] = rd4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kc = 0; kc < KG; ++kc) {
+        const int r = kc / 4, jj = kc % 4;
+        const int i = (r % RT) * 16 + 4 * jj + g;
         const double* Wt = r < RT ? WA : WB;
-        const double w = Wt[min(i, M - 1) * M + min(j, M - 1)];
-        ay[r] = __builtin_amdgcn_mfma_f64_16x16x4f64((i < M && j < M) ? w : 0.0, bx, ay[r], 0, 0, 0);
+        const double bh = ay[r][jj];
+#pragma unroll
+        for (int d = 0; d < RT; ++d) {
+          const int dof = d * 16 + m;
+          const double w = Wt[min(i, M - 1) * M + min(dof, M - 1)];
+          ag[d] = __builtin_amdgcn_mfma_f64_16x16x4f64((i < M && dof < M) ? w : 0.0, bh, ag[d],
+                                                       0, 0, 0);
+        }
       }
+      if (g == 0) vpot[tb + m] = (0.5 * (va + vb) - R) - A.pes.v0;  // as eval_kind
+#pragma unroll
+      for (int d = 0; d < RT; ++d)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) gp[(tb + m) * LX + d * 16 + g + 4 * jj] = ag[d][jj];
     }
-    double va = 0.0, vb = 0.0;
+    lds_sync<NT>();
+    // springs per (bead, dof), as in ring_um_kernel; the neighbours, masses and end beads come
+    // from LDS
+    const double bn2 = A.betan[c] * A.betan[c];
+    const double* a = sm + D;
+    const double* b = sm + 2 * D;
+    double e = 0.0;
+    double* __restrict__ gout = grad ? grad + A.off[c] : nullptr;  // loaded once, not per store
+    const bool fixedends = A.fixedends;
+    constexpr int NSP = (TB * M + NT - 1) / NT;
 #pragma unroll
-    for (int r = 0; r < 2 * RT; ++r)
-#pragma unroll
-      for (int jj = 0; jj < 4; ++jj) {
-        const int i = (r % RT) * 16 + g + 4 * jj;
-        const bool ok = i < M;
-        const int ic = min(i, M - 1);
-        const double cw = ok ? (r < RT ? cA[ic] : cB[ic]) : 0.0;
-        const double li = ok ? lam[ic] : 0.0, ci = ok ? cub[ic] : 0.0;
-        const double y = ay[r][jj] - cw, y2 = y * y;
-        const double v = (0.5 * li) * y2 + ci * y2 * y;
-        if (r < RT)
-          va += v;
-        else
-          vb += v;
-        ay[r][jj] = li * y + (3.0 * ci) * y2;
+    for (int q = 0; q < NSP; ++q) {
+      const int t = tid + q * NT;
+      const int d = t / TB, bl = t - d * TB;
+      const int i = b0 + bl;
+      if (t >= TB * D || i >= n) continue;
+      const double m_ = sm[d];
+      const double xi = xs[(bl + 1) * LX + d];
+      const double xl = xs[bl * LX + d];        // bead i - 1 (0 before the first bead)
+      const double xr = xs[(bl + 2) * LX + d];  // bead i + 1 (0 after the last bead)
+      if (d == 0) e += vpot[bl];
+      if (i < n - 1) {
+        const double dx = xr - xi;
+        e += (0.5 * m_ / bn2) * dx * dx;
       }
-    va += __shfl_xor(va, 16);
-    va += __shfl_xor(va, 32);
-    vb += __shfl_xor(vb, 16);
-    vb += __shfl_xor(vb, 32);
-    const double dl = va - vb, R = sqrt(0.25 * dl * dl + kappa * kappa), f = dl / (4.0 * R);
-    const double wa = 0.5 - f, wb = 0.5 + f;
-#pragma unroll
-    for (int r = 0; r < 2 * RT; ++r) ay[r] *= (r < RT ? wa : wb);
-    rd4 ag[RT];
-#pragma unroll
-    for (int d = 0; d < RT; ++d) ag[d] = rd4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int kc = 0; kc < KG; ++kc) {
-      const int r = kc / 4, jj = kc % 4;
-      const int i = (r % RT) * 16 + 4 * jj + g;
-      const double* Wt = r < RT ? WA : WB;
-      const double bh = ay[r][jj];
-#pragma unroll
-      for (int d = 0; d < RT; ++d) {
-        const int dof = d * 16 + m;
-        const double w = Wt[min(i, M - 1) * M + min(dof, M - 1)];
-        ag[d] = __builtin_amdgcn_mfma_f64_16x16x4f64((i < M && dof < M) ? w : 0.0, bh, ag[d], 0, 0, 0);
-      }
+      double sg;
+      if (n == 1)
+        sg = fixedends ? m_ * (2.0 * xi - a[d] - b[d]) / bn2 : 0.0;
+      else if (i == 0)
+        sg = fixedends ? m_ * (2.0 * xi - a[d] - xr) / bn2 : m_ * (xi - xr) / bn2;
+      else if (i == n - 1)
+        sg = fixedends ? m_ * (2.0 * xi - xl - b[d]) / bn2 : m_ * (xi - xl) / bn2;
+      else
+        sg = m_ * (2.0 * xi - xl - xr) / bn2;
+      if (fixedends && i == 0) e += (0.5 * m_ / bn2) * (xi - a[d]) * (xi - a[d]);
+      if (fixedends && i == n - 1) e += (0.5 * m_ / bn2) * (b[d] - xi) * (b[d] - xi);
+      if (gout) gout[(size_t)d * n + i] = sg + gp[bl * LX + d];
     }
-    if (g == 0) vpot[tb + m] = (0.5 * (va + vb) - R) - A.pes.v0;  // as eval_kind
+    // the energy: a wave sum, then the waves' sums through LDS behind an LDS-only barrier (a
+    // full barrier would first wait for every gradient store above to be acknowledged)
 #pragma unroll
-    for (int d = 0; d < RT; ++d)
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o);
+    if (lane == 0) red[wave] = e;
+    lds_sync<NT>();
+    if (tid == 0) {
+      double t = 0.0;
 #pragma unroll
-      for (int jj = 0; jj < 4; ++jj) gp[(tb + m) * LX + d * 16 + g + 4 * jj] = ag[d][jj];
-  }
-  __syncthreads();
-  // springs per (bead, dof), as in ring_um_kernel; the neighbours come from the LDS rows
-  const double bn2 = A.betan[c] * A.betan[c];
-  const double* a = A.ab + (size_t)c * 2 * D;
-  const double* b = a + D;
-  double e = 0.0;
-  for (int t = tid; t < TB * D; t += NT) {
-    const int d = t / TB, bl = t - d * TB;
-    const int i = b0 + bl;
-    if (i >= n) continue;
-    const double m_ = A.massd[d];
-    const double xi = xs[(bl + 1) * LX + d];
-    const double xl = xs[bl * LX + d];        // bead i - 1 (0 before the first bead)
-    const double xr = xs[(bl + 2) * LX + d];  // bead i + 1 (0 after the last bead)
-    if (d == 0) e += vpot[bl];
-    if (i < n - 1) {
-      const double dx = xr - xi;
-      e += (0.5 * m_ / bn2) * dx * dx;
+      for (int w = 0; w < kUmWaves; ++w) t += red[w];
+      atomicAdd(&energy[c], t);
     }
-    double sg;
-    if (n == 1)
-      sg = A.fixedends ? m_ * (2.0 * xi - a[d] - b[d]) / bn2 : 0.0;
-    else if (i == 0)
-      sg = A.fixedends ? m_ * (2.0 * xi - a[d] - xr) / bn2 : m_ * (xi - xr) / bn2;
-    else if (i == n - 1)
-      sg = A.fixedends ? m_ * (2.0 * xi - xl - b[d]) / bn2 : m_ * (xi - xl) / bn2;
-    else
-      sg = m_ * (2.0 * xi - xl - xr) / bn2;
-    if (A.fixedends && i == 0) e += (0.5 * m_ / bn2) * (xi - a[d]) * (xi - a[d]);
-    if (A.fixedends && i == n - 1) e += (0.5 * m_ / bn2) * (b[d] - xi) * (b[d] - xi);
-    if (grad) grad[A.off[c] + (size_t)d * n + i] = sg + gp[bl * LX + d];
-  }
-  red[tid] = e;
-  __syncthreads();
-  for (int st = NT / 2; st > 0; st >>= 1) {
-    if (tid < st) red[tid] += red[tid + st];
-    __syncthreads();
-  }
-  if (tid == 0) atomicAdd(&energy[c], red[0]);
+  }  // tasks
 }
 
 // Hessian-vector product of U_M: springs exactly, potential by forward-over-reverse.
@@ -466,21 +531,6 @@ constexpr int kMaxD = 56;  // ndof bound of the log-determinant workspace (LDS)
 // of eliminations): the pivot of elimination k is the k-th pivot of the matrix's LDL^T
 // factorisation (a ratio of leading principal minors), so their logs sum to log|det| and
 // their signs give the inertia (Sylvester).
-// Barrier of ring_logdet_kernel: LDS ordering only, so the global loads of the next block
-// stay in flight across it (a one-wave workgroup needs no hardware barrier at all: its LDS
-// operations complete in issue order).
-template <int T>
-__device__ __forceinline__ void lds_sync() {
-  if constexpr (T == 64) {
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-  } else {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-  }
-}
-
 // T threads per chain.  Thread t owns the elements e = t + T q (q < P) of every D x D block,
 // so the Hessian block of the next bead is loaded into registers while the current one is
 // eliminated, and the Schur update uses the previous inverse where it sits in LDS.  The
@@ -841,6 +891,9 @@ struct rp_ring {
   double pc_sigma = 0.0;
   DevBuf<double> d_pctab;
   DevBuf<int64_t> d_pcoff;
+  // the EVB U_M kernel's bead tiles (chain, first bead) and its persistent grid
+  DevBuf<int> d_umtask;
+  int n_umtask = 0, um_grid = 0;
 };
 
 static RingArgs ring_args(rp_ring* r) {
@@ -898,9 +951,26 @@ int ring_eval(rp_ring* r, bool want_grad) {
   return dispatch_pes(r->pes, [&]<int K, int M>() -> int {
     if constexpr (K == PES_EVB) {  // the EVB's two GEMMs per bead tile on the matrix cores
       constexpr int TB = kUmWaves * 16;
-      const dim3 g2((unsigned)((r->nmax + TB - 1) / TB), (unsigned)r->nchain);
-      ring_um_evb_kernel<M><<<g2, kUmWaves * 64, 0, r->s>>>(a, r->d_x.p,
-                                                              want_grad ? r->d_g.p : nullptr, r->d_e.p);
+      if (!r->d_umtask.p) {  // the bead tiles of every chain, once per ring
+        std::vector<int> t;
+        for (int c = 0; c < r->nchain; ++c)
+          for (int b0 = 0; b0 < r->n[c]; b0 += TB) {
+            t.push_back(c);
+            t.push_back(b0);
+          }
+        int rc;
+        if ((rc = r->d_umtask.alloc(t.size())) || (rc = r->d_umtask.upload(t.data(), t.size(), r->s)))
+          return rc;
+        r->n_umtask = (int)(t.size() / 2);
+        int ncu = 0, dev = 0, per_cu = 0;
+        RP_HIP(hipGetDevice(&dev));
+        RP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        RP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ring_um_evb_kernel<M>,
+                                                            kUmWaves * 64, 0));
+        r->um_grid = std::min(r->n_umtask, std::max(per_cu, 1) * std::max(ncu, 1));
+      }
+      ring_um_evb_kernel<M><<<r->um_grid, kUmWaves * 64, 0, r->s>>>(
+          a, r->d_x.p, want_grad ? r->d_g.p : nullptr, r->d_e.p, r->d_umtask.p, r->n_umtask);
     } else {
       ring_um_kernel<K, M><<<grid, kRingTile, 0, r->s>>>(a, r->d_x.p,
                                                                 want_grad ? r->d_g.p : nullptr, r->d_e.p);
@@ -1034,12 +1104,14 @@ extern "C" int rp_ring_destroy(rp_ring* r) {
   if (r->s) (void)hipStreamSynchronize(r->s);
   for (DevBuf<double>* b : {&r->d_x, &r->d_g, &r->d_v, &r->d_hv, &r->d_e, &r->d_ab, &r->d_betan,
                             &r->d_massd, &r->d_blocks, &r->d_sinv, &r->d_work, &r->d_ld, &r->d_eta,
-                            &r->d_pe})
+                            &r->d_pe, &r->d_pctab})
     b->free();
   r->d_n.free();
   r->d_nneg.free();
+  r->d_umtask.free();
   r->d_off.free();
   r->d_boff.free();
+  r->d_pcoff.free();
   if (r->s) (void)hipStreamDestroy(r->s);
   delete r;
   return RP_OK;
