@@ -1313,7 +1313,7 @@ int ring_lowest_eigen(rp_ring* r, const std::vector<int>& ng, double* eta) {
   std::vector<double> mu(C), sh(C, 0.0);
   std::vector<int> act(C);
   // inverse iteration on the unshifted factors, from a fixed generic start
-  if ((rc = eta1(nullptr, 12, 1, 0.0, nullptr)) || (rc = fetch_eta(mu))) return cleanup(), rc;
+  if ((rc = eta1(nullptr, 6, 1, 0.0, nullptr)) || (rc = fetch_eta(mu))) return cleanup(), rc;
   // chains with negative eigenvalues: Rayleigh-quotient refinement toward the one nearest zero
   bool any = false;
   for (int c = 0; c < C; ++c) any |= (act[c] = ng[c] != 0);
