@@ -1292,7 +1292,8 @@ int ring_lowest_eigen(rp_ring* r, const std::vector<int>& ng, double* eta) {
   DevBuf<int> d_act, d_map, d_cnt;
   auto cleanup = [&]() { d_shift.free(); d_tld.free(); d_act.free(); d_map.free(); d_cnt.free(); };
   if ((rc = d_shift.alloc(M * (size_t)C)) || (rc = d_tld.alloc(M * (size_t)C)) ||
-      (rc = d_act.alloc(C)) || (rc = d_map.alloc(M * (size_t)C)) || (rc = d_cnt.alloc(M * (size_t)C))) {
+      (rc = d_act.alloc(C)) || (rc = d_map.alloc(M * (size_t)C)) ||
+      (rc = d_cnt.alloc(M * (size_t)C))) {
     cleanup();
     return rc;
   }
@@ -1307,7 +1308,8 @@ int ring_lowest_eigen(rp_ring* r, const std::vector<int>& ng, double* eta) {
   };
   auto fetch_eta = [&](std::vector<double>& e) -> int {
     int q = r->d_eta.download(e.data(), C, r->s);
-    if (!q && hipStreamSynchronize(r->s) != hipSuccess) q = set_error(RP_ERR_HIP, "inverse iteration");
+    if (!q && hipStreamSynchronize(r->s) != hipSuccess)
+      q = set_error(RP_ERR_HIP, "inverse iteration");
     return q;
   };
   std::vector<double> mu(C), sh(C, 0.0);
@@ -1339,7 +1341,9 @@ int ring_lowest_eigen(rp_ring* r, const std::vector<int>& ng, double* eta) {
     eta[c] = mu[c];
     if (ng[c] != 0) continue;
     if (!(mu[c] > 0.0))
-      return cleanup(), set_error(RP_ERR_CONVERGE, "chain %d: inverse iteration gave eta = %g with no negative eigenvalue", c, mu[c]);
+      return cleanup(), set_error(RP_ERR_CONVERGE,
+                                  "chain %d: inverse iteration gave eta = %g with no negative "
+                                  "eigenvalue", c, mu[c]);
     hi[c] = mu[c] * (1.0 + 1e-13);
     open[c] = 1;
   }
@@ -1367,7 +1371,8 @@ int ring_lowest_eigen(rp_ring* r, const std::vector<int>& ng, double* eta) {
         (rc = ring_factor(r, (int)map.size(), d_tld.p, d_map.p, false, d_shift.p, d_cnt.p)) ||
         (rc = d_cnt.download(cnt.data(), cnt.size(), r->s)))
       return cleanup(), rc;
-    if (hipStreamSynchronize(r->s) != hipSuccess) return cleanup(), set_error(RP_ERR_HIP, "inertia count");
+    if (hipStreamSynchronize(r->s) != hipSuccess)
+      return cleanup(), set_error(RP_ERR_HIP, "inertia count");
     // the counts rise with the shift: the bracket closes onto the first rise
     for (size_t t = 0; t < map.size(); ++t) {
       const int c = map[t];
@@ -1402,7 +1407,9 @@ int ring_lowest_eigen(rp_ring* r, const std::vector<int>& ng, double* eta) {
       return cleanup(), rc;
     for (int c : map) {
       if (!(e[c] > lo[c] && e[c] <= hi[c] * (1.0 + 1e-12)))
-        return cleanup(), set_error(RP_ERR_CONVERGE, "chain %d: eta_1 = %.17g left its inertia bracket [%.17g, %.17g]", c, e[c], lo[c], hi[c]);
+        return cleanup(), set_error(RP_ERR_CONVERGE,
+                                    "chain %d: eta_1 = %.17g left its inertia bracket "
+                                    "[%.17g, %.17g]", c, e[c], lo[c], hi[c]);
       eta[c] = e[c];
     }
   }
