@@ -254,8 +254,11 @@ int rp_ring_hessian_band(rp_ring *r, int chain, int singlewell, double *band);
  * (ascending, instantonmod.f90:782-827): logdet_pos[c] = sum of ln(eta) over eta > 0,
  * skipping eta_1 (the lowest) when skip_lowest (lndetJ, rpi_ser.f90:355-365), else over
  * every eta > 0 (lndetJ0, :227-236); nneg[c] = number of negative eigenvalues (the inertia);
- * eta1[c] = the lowest eigenvalue.  Negative eigenvalues beyond the one inverse iteration
- * finds are located by bisection on the inertia, so any spectrum is reduced. */
+ * eta1[c] = the lowest eigenvalue.  With no negative eigenvalue it is held on a bracket of
+ * inertia counts and converged by inverse iteration shifted below it, so a cluster at the
+ * bottom of the spectrum cannot trade it for a neighbour; negative eigenvalues beyond the
+ * one inverse iteration finds are located by bisection on the inertia, so any spectrum is
+ * reduced. */
 int rp_ring_logdet(rp_ring *r, int singlewell, int skip_lowest, double *logdet_pos,
                    int *nneg, double *eta1);
 /* the k lowest eigenvalues of every chain's mass-weighted ring Hessian (the band of

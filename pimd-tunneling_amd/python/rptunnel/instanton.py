@@ -113,9 +113,10 @@ class Ring:
 
     def logdet(self, singlewell=False, skip_lowest=True, want_eta=None):
         """-> (sum of ln(eta) over the eigenvalues the reference keeps, nneg, eta_1).
-        eta_1 (the lowest eigenvalue: inverse iteration finds the one nearest zero, bisection
-        on the inertia any other negative ones) is computed when skip_lowest needs it or
-        want_eta asks for it; otherwise it is returned as NaN."""
+        eta_1 (the lowest eigenvalue: with none negative, inverse iteration shifted below it
+        on a bracket of inertia counts; else the one nearest zero by inverse iteration and
+        any other negative ones by bisection on the inertia) is computed when skip_lowest
+        needs it or want_eta asks for it; otherwise it is returned as NaN."""
         if want_eta is None:
             want_eta = skip_lowest
         ld = np.zeros(self.nchain)
