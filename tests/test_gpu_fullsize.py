@@ -1,0 +1,91 @@
+"""The batched TI workload at its full size, on the launch shape it is timed with: the 27-dof
+EVB double well, N in {32, 64, 96, 128, 192, 256}, 512 trajectories x 11 lambda nodes at
+every N (33,792 polymers), Langevin, the engine's own Philox streams after init_path, 8 steps
+per launch from captured graphs (propagate_pimd_pile, verletmodule.f90:372-416).
+
+Checked without shrinking it:
+* the oracle replays the first and the last polymer of every bead-count family from the
+  state init_path gave them and the stream position it left: states and TI averages 1e-10
+  normwise relative, stream cursors exact;
+* a second batch built the same way gives bit-identical TI sums, counts and cursors for every
+  polymer, and bit-identical states for the sampled ones (one workgroup per polymer, no
+  atomics: the run is deterministic);
+* every TI sum is finite and every polymer counted the same number of steps.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import oracle_pes, rel
+from rptunnel import Batch, Pes, specs
+
+pytestmark = pytest.mark.gpu
+
+NS = [32, 64, 96, 128, 192, 256]
+NTRAJ, NLAM = 512, 11
+BETA, TAU, DT, GAMMA, SEED = 2000.0, 20.0, 1.0, 1e-3, 2024
+STEPS, IMIN = 16, 4
+
+
+def _polymers(sp):
+    t, _ = np.polynomial.legendre.leggauss(NLAM)
+    xi = 0.5 * (t + 1.0)  # Gauss-Legendre nodes on [0, 1]
+    pols, stream = [], 0
+    for n in NS:
+        for x in xi:
+            b = sp.well1 + x * (sp.well2 - sp.well1)
+            for _ in range(NTRAJ):
+                pols.append(dict(n=n, betan=BETA / (n + 1), xi=float(x), a=sp.well1, b=b,
+                                 dbdl=sp.well2 - sp.well1, stream=stream))
+                stream += 1
+    return pols
+
+
+def _batch(pes, sp, pols):
+    b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, seed=SEED)
+    lampath, path = specs.linear_path(sp.well1, sp.well2, npath=2)
+    b.set_path(lampath, path)
+    b.init_path()
+    b.set_launch(steps_per_launch=8)
+    return b
+
+
+def test_full_workload_against_oracle_and_deterministic():
+    sp = specs.evb()
+    pes = Pes(sp)
+    op = oracle_pes(sp)
+    pols = _polymers(sp)
+    per = NTRAJ * NLAM
+    assert len(pols) == len(NS) * per == 33792
+    sample = [k for f in range(len(NS)) for k in (f * per, (f + 1) * per - 1)]
+    b = _batch(pes, sp, pols)
+    start = {k: b.get_state(k) for k in sample}
+    cur0 = b.cursors()
+    b.run(STEPS, imin=IMIN)
+    sums, cnt, _ = b.ti_readout()
+    cur = b.cursors()
+    assert np.all(cnt == STEPS - IMIN) and np.all(np.isfinite(sums))
+    for k in sample:
+        pol = pols[k]
+        nm = O.NM(pol["n"], sp.ndim, sp.natom, sp.mass, pol["betan"], TAU, DT, GAMMA, False,
+                  pol["a"], pol["b"])
+        r = O.Rng(seed=SEED, stream=pol["stream"])
+        r.s.cursor = int(cur0[k])
+        x0, p0 = start[k]
+        xo, po, dho, _ = O.propagate_pile(nm, op, r, x0, p0, STEPS, pol["dbdl"], imin=IMIN)
+        xg, pg = b.get_state(k)
+        assert cur[k] == r.cursor, (k, cur[k], r.cursor)
+        assert rel(xg, xo) <= 1e-10, (k, pol["n"], rel(xg, xo))
+        assert rel(pg, po) <= 1e-10, (k, pol["n"], rel(pg, po))
+        assert abs(sums[k] / (STEPS - IMIN) - dho) <= 1e-10 * abs(dho), (k, sums[k], dho)
+    states = {k: b.get_state(k) for k in sample}
+    b.close()
+    b2 = _batch(pes, sp, pols)
+    b2.run(STEPS, imin=IMIN)
+    sums2, cnt2, _ = b2.ti_readout()
+    assert np.array_equal(sums2, sums) and np.array_equal(cnt2, cnt)
+    assert np.array_equal(b2.cursors(), cur)
+    for k in sample:
+        x2, p2 = b2.get_state(k)
+        assert np.array_equal(x2, states[k][0]) and np.array_equal(p2, states[k][1]), k
+    b2.close()
