@@ -1,7 +1,8 @@
-"""The batched TI workload at its full size, on the launch shape it is timed with: the 27-dof
-EVB double well, N in {32, 64, 96, 128, 192, 256}, 512 trajectories x 11 lambda nodes at
-every N (33,792 polymers), Langevin, the engine's own Philox streams after init_path, 8 steps
-per launch from captured graphs (propagate_pimd_pile, verletmodule.f90:372-416).
+"""The batched TI workloads at their full sizes, on the launch shape they are timed with: the
+27-dof EVB double well, N in {32, 64, 96, 128, 192, 256}, 512 trajectories x 11 lambda nodes
+at every N (33,792 polymers), and the 6-water cluster (54 dof), N = 128, 1024 trajectories x
+11 lambda nodes (11,264 polymers); Langevin, the engine's own Philox streams after init_path,
+8 steps per launch from captured graphs (propagate_pimd_pile, verletmodule.f90:372-416).
 
 Checked without shrinking it:
 * the oracle replays the first and the last polymer of every bead-count family from the
@@ -17,48 +18,65 @@ import pytest
 
 import oracle as O
 from helpers import oracle_pes, rel
-from rptunnel import Batch, Pes, specs
+from rptunnel import Batch, Pes, specs, ti
 
 pytestmark = pytest.mark.gpu
 
-NS = [32, 64, 96, 128, 192, 256]
-NTRAJ, NLAM = 512, 11
+NLAM = 11
 BETA, TAU, DT, GAMMA, SEED = 2000.0, 20.0, 1.0, 1e-3, 2024
 STEPS, IMIN = 16, 4
+# (potential, bead counts, trajectories per lambda node and bead count, polymers)
+WORKLOADS = [("evb27", [32, 64, 96, 128, 192, 256], 512, 33792),
+             ("water6", [128], 1024, 11264)]
 
 
-def _polymers(sp):
+def _spec(name):
+    return specs.evb() if name == "evb27" else specs.water_cluster(nmol=6)
+
+
+def _path(name, sp):
+    """(lampath, path (ndof, npath)): the straight line between the wells for the EVB; for
+    the water cluster the path that turns molecule 0's H pair about its bisector (a straight
+    line to the exchanged well would run the two H atoms through each other)."""
+    if name == "evb27":
+        return specs.linear_path(sp.well1, sp.well2, npath=2)
+    path = np.ascontiguousarray(sp.extra["path"].T)
+    return specs.arc_lampath(path), path
+
+
+def _polymers(sp, ns, ntraj, lampath, path):
     t, _ = np.polynomial.legendre.leggauss(NLAM)
     xi = 0.5 * (t + 1.0)  # Gauss-Legendre nodes on [0, 1]
+    ends, grads = ti._spline_nodes(lampath, path, xi)  # the path's point and slope at xi
     pols, stream = [], 0
-    for n in NS:
-        for x in xi:
-            b = sp.well1 + x * (sp.well2 - sp.well1)
-            for _ in range(NTRAJ):
-                pols.append(dict(n=n, betan=BETA / (n + 1), xi=float(x), a=sp.well1, b=b,
-                                 dbdl=sp.well2 - sp.well1, stream=stream))
+    for n in ns:
+        for il, x in enumerate(xi):
+            for _ in range(ntraj):
+                pols.append(dict(n=n, betan=BETA / (n + 1), xi=float(x), a=path[:, 0],
+                                 b=ends[il], dbdl=grads[il], stream=stream))
                 stream += 1
     return pols
 
 
-def _batch(pes, sp, pols):
+def _batch(pes, sp, pols, lampath, path):
     b = Batch(pes, sp.mass, pols, TAU, DT, GAMMA, seed=SEED)
-    lampath, path = specs.linear_path(sp.well1, sp.well2, npath=2)
     b.set_path(lampath, path)
     b.init_path()
     b.set_launch(steps_per_launch=8)
     return b
 
 
-def test_full_workload_against_oracle_and_deterministic():
-    sp = specs.evb()
+@pytest.mark.parametrize("name,ns,ntraj,npoly", WORKLOADS)
+def test_full_workload_against_oracle_and_deterministic(name, ns, ntraj, npoly):
+    sp = _spec(name)
     pes = Pes(sp)
     op = oracle_pes(sp)
-    pols = _polymers(sp)
-    per = NTRAJ * NLAM
-    assert len(pols) == len(NS) * per == 33792
-    sample = [k for f in range(len(NS)) for k in (f * per, (f + 1) * per - 1)]
-    b = _batch(pes, sp, pols)
+    lampath, path = _path(name, sp)
+    pols = _polymers(sp, ns, ntraj, lampath, path)
+    per = ntraj * NLAM
+    assert len(pols) == len(ns) * per == npoly
+    sample = [k for f in range(len(ns)) for k in (f * per, (f + 1) * per - 1)]
+    b = _batch(pes, sp, pols, lampath, path)
     start = {k: b.get_state(k) for k in sample}
     cur0 = b.cursors()
     b.run(STEPS, imin=IMIN)
@@ -80,7 +98,7 @@ def test_full_workload_against_oracle_and_deterministic():
         assert abs(sums[k] / (STEPS - IMIN) - dho) <= 1e-10 * abs(dho), (k, sums[k], dho)
     states = {k: b.get_state(k) for k in sample}
     b.close()
-    b2 = _batch(pes, sp, pols)
+    b2 = _batch(pes, sp, pols, lampath, path)
     b2.run(STEPS, imin=IMIN)
     sums2, cnt2, _ = b2.ti_readout()
     assert np.array_equal(sums2, sums) and np.array_equal(cnt2, cnt)
