@@ -1,8 +1,10 @@
 """The batched TI workloads at their full sizes, on the launch shape they are timed with: the
 27-dof EVB double well, N in {32, 64, 96, 128, 192, 256}, 512 trajectories x 11 lambda nodes
-at every N (33,792 polymers), and the 6-water cluster (54 dof), N = 128, 1024 trajectories x
-11 lambda nodes (11,264 polymers); Langevin, the engine's own Philox streams after init_path,
-8 steps per launch from captured graphs (propagate_pimd_pile, verletmodule.f90:372-416).
+at every N (33,792 polymers); the 2D coupled double well, N = 128, 4096 trajectories x 16
+lambda nodes (65,536 polymers); the 6-water cluster (54 dof), N = 128, 1024 trajectories x
+11 lambda nodes (11,264 polymers).  Langevin, the engine's own Philox streams after
+init_path, 8 steps per launch from captured graphs (propagate_pimd_pile,
+verletmodule.f90:372-416).
 
 Checked without shrinking it:
 * the oracle replays the first and the last polymer of every bead-count family from the
@@ -22,30 +24,32 @@ from rptunnel import Batch, Pes, specs, ti
 
 pytestmark = pytest.mark.gpu
 
-NLAM = 11
 BETA, TAU, DT, GAMMA, SEED = 2000.0, 20.0, 1.0, 1e-3, 2024
 STEPS, IMIN = 16, 4
-# (potential, bead counts, trajectories per lambda node and bead count, polymers)
-WORKLOADS = [("evb27", [32, 64, 96, 128, 192, 256], 512, 33792),
-             ("water6", [128], 1024, 11264)]
+# (potential, bead counts, trajectories per lambda node and bead count, lambda nodes,
+# polymers)
+WORKLOADS = [("evb27", [32, 64, 96, 128, 192, 256], 512, 11, 33792),
+             ("coupled2d", [128], 4096, 16, 65536),
+             ("water6", [128], 1024, 11, 11264)]
 
 
 def _spec(name):
-    return specs.evb() if name == "evb27" else specs.water_cluster(nmol=6)
+    return {"evb27": specs.evb, "coupled2d": specs.coupled2d,
+            "water6": lambda: specs.water_cluster(nmol=6)}[name]()
 
 
 def _path(name, sp):
     """(lampath, path (ndof, npath)): the straight line between the wells for the EVB; for
     the water cluster the path that turns molecule 0's H pair about its bisector (a straight
     line to the exchanged well would run the two H atoms through each other)."""
-    if name == "evb27":
+    if name != "water6":
         return specs.linear_path(sp.well1, sp.well2, npath=2)
     path = np.ascontiguousarray(sp.extra["path"].T)
     return specs.arc_lampath(path), path
 
 
-def _polymers(sp, ns, ntraj, lampath, path):
-    t, _ = np.polynomial.legendre.leggauss(NLAM)
+def _polymers(sp, ns, ntraj, nlam, lampath, path):
+    t, _ = np.polynomial.legendre.leggauss(nlam)
     xi = 0.5 * (t + 1.0)  # Gauss-Legendre nodes on [0, 1]
     ends, grads = ti._spline_nodes(lampath, path, xi)  # the path's point and slope at xi
     pols, stream = [], 0
@@ -66,14 +70,14 @@ def _batch(pes, sp, pols, lampath, path):
     return b
 
 
-@pytest.mark.parametrize("name,ns,ntraj,npoly", WORKLOADS)
-def test_full_workload_against_oracle_and_deterministic(name, ns, ntraj, npoly):
+@pytest.mark.parametrize("name,ns,ntraj,nlam,npoly", WORKLOADS)
+def test_full_workload_against_oracle_and_deterministic(name, ns, ntraj, nlam, npoly):
     sp = _spec(name)
     pes = Pes(sp)
     op = oracle_pes(sp)
     lampath, path = _path(name, sp)
-    pols = _polymers(sp, ns, ntraj, lampath, path)
-    per = ntraj * NLAM
+    pols = _polymers(sp, ns, ntraj, nlam, lampath, path)
+    per = ntraj * nlam
     assert len(pols) == len(ns) * per == npoly
     sample = [k for f in range(len(ns)) for k in (f * per, (f + 1) * per - 1)]
     b = _batch(pes, sp, pols, lampath, path)
