@@ -961,6 +961,7 @@ int ring_eval(rp_ring* r, bool want_grad) {
         int rc;
         if ((rc = r->d_umtask.alloc(t.size())) || (rc = r->d_umtask.upload(t.data(), t.size(), r->s)))
           return rc;
+        RP_HIP(hipStreamSynchronize(r->s));  // t is released at the end of this block
         r->n_umtask = (int)(t.size() / 2);
         int ncu = 0, dev = 0, per_cu = 0;
         RP_HIP(hipGetDevice(&dev));
