@@ -162,8 +162,8 @@ def _mkl_dsymv():
     return ctypes.cast(ctypes.CDLL(MKL_RT).dsymv_, ctypes.c_void_p).value
 
 
-def config1_sample(threads, nrep=8, nsteps=100000, imin=1000, n=64):
-    """Config 1 in full: the 1D quartic double well (V0 = 0.005, a = 0.7, m = 1836), N = 64,
+def quartic_full_sample(threads, nrep=8, nsteps=100000, imin=1000, n=64):
+    """The 1D quartic double well in full (V0 = 0.005, a = 0.7, m = 1836), N = 64,
     beta = 2000, 11 Gauss-Legendre nodes x nrep replicas x nsteps Langevin steps, on the GPU
     (the batched engine: init_path, TI after imin, the reference driver's quadrature) and
     on the host cores (the oracle's restatement through MKL dsymv, from the straight line
@@ -224,10 +224,10 @@ def config1_sample(threads, nrep=8, nsteps=100000, imin=1000, n=64):
                                           / math.sqrt(g["sigmaA"] / nrep + c["sigmaA"] / nrep)}
 
 
-def cpu_configs_sample(threads, budget_s=3.0):
-    """The host-core baselines of the other configurations (oracle restatement; MKL dsymv
-    for the transforms): config 2 and config 5 bead-steps/s on a bounded window of all
-    cores (as the config-3 baseline), and config 4's instanton figures at N = 64, 256, 1024:
+def cpu_workloads_sample(threads, budget_s=3.0):
+    """The host-core baselines of the other workloads (oracle restatement; MKL dsymv for the
+    transforms): coupled-2D and water-cluster bead-steps/s on a bounded window of all cores
+    (as the EVB TI baseline), and the EVB instanton figures at N = 64, 256, 1024:
     U_M + gradient bead-evaluations/s (one chain per thread), the UMhessian band builds and
     the DSBEVD eigenvalue solve (LAPACK through scipy's eig_banded, the call detJ makes)."""
     from concurrent.futures import ThreadPoolExecutor
@@ -238,8 +238,8 @@ def cpu_configs_sample(threads, budget_s=3.0):
     from rptunnel import specs
 
     out = {}
-    for name, sp in (("config2_coupled2d", specs.coupled2d()),
-                     ("config5_water6", specs.water_cluster(nmol=6))):
+    for name, sp in (("coupled2d", specs.coupled2d()),
+                     ("water6", specs.water_cluster(nmol=6))):
         tc, detail, symv = cpu_sample(sp, [128], budget_s, threads)
         out[name] = {"bead_steps_per_s": 128 / tc[128], "sample": detail[128], "symv": symv}
     sp = specs.evb()
@@ -275,7 +275,7 @@ def cpu_configs_sample(threads, budget_s=3.0):
         inst[n] = {"um_grad_bead_evals_per_s": threads * reps * n / tu,
                    "hessian_band_bead_blocks_per_s": n / th,
                    "dsbevd_bead_blocks_per_s": n / te if te else None}
-    out["config4_evb_instantons"] = {"per_N": inst, "threads_for_um": threads,
+    out["evb_instantons"] = {"per_N": inst, "threads_for_um": threads,
                                      "hessian_and_eigensolve": "one core each; the eigensolve "
                                      "only up to N = 256"}
     return out
@@ -466,7 +466,7 @@ def water_sample(steps=16, warmup=8, ntraj=1024, n=128, nchain=256):
 
 
 def coupled2d_sample(steps=16, warmup=8, ntraj=4096, nlam=16, n=128, plain_steps=20):
-    """Config 2: the 2D coupled double well (V0 = 0.005, a = 0.7, w = 0.01, c = 0.3, mass
+    """The 2D coupled double well (V0 = 0.005, a = 0.7, w = 0.01, c = 0.3, mass
     1836), PIMD-TI at N = 128 with ntraj trajectories x nlam lambda nodes (65,536 polymers),
     Langevin, the headline's beta / tau / dt / gamma, on the batched engine (inputs resident,
     device-timed) and on the plain per-trajectory path (one trajectory, plain_steps steps)."""
@@ -587,10 +587,11 @@ def main():
                     help="skip the same-TI-on-both-paths sample reported beside the headline")
     ap.add_argument("--no-andersen", action="store_true",
                     help="skip the Andersen-thermostat sample reported beside the headline")
-    ap.add_argument("--no-configs", action="store_true",
-                    help="skip config 1 in full and the host baselines of configs 2, 4 and 5")
-    ap.add_argument("--no-config2", action="store_true",
-                    help="skip the coupled-2D (config 2) sample reported beside the headline")
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip the quartic TI in full and the host baselines of the coupled-2D, "
+                         "EVB-instanton and water workloads")
+    ap.add_argument("--no-coupled2d", action="store_true",
+                    help="skip the coupled-2D sample reported beside the headline")
     ap.add_argument("--no-water", action="store_true",
                     help="skip the water-cluster sample reported beside the headline")
     ap.add_argument("--steps-per-launch", type=int, default=8,
@@ -681,23 +682,23 @@ def main():
         log("instanton sample (256 chains)")
         inst = instanton_sample(pes, sp)
 
-    config2 = None
-    if not args.no_config2:
+    coupled = None
+    if not args.no_coupled2d:
         log("coupled 2D sample (N = 128, 65536 polymers)")
-        config2 = coupled2d_sample()
+        coupled = coupled2d_sample()
 
     water = None
     if not args.no_water:
         log("water-cluster sample (N = 128, 11264 polymers)")
         water = water_sample()
 
-    config1 = cpu_configs = None
-    if not args.no_cpu and not args.no_configs:
+    quartic = cpu_others = None
+    if not args.no_cpu and not args.no_others:
         threads = host_cores()
-        log(f"config 1 in full on the GPU and on {threads} host threads")
-        config1 = config1_sample(threads)
-        log("host baselines of configs 2, 4 and 5")
-        cpu_configs = cpu_configs_sample(threads)
+        log(f"the quartic TI in full on the GPU and on {threads} host threads")
+        quartic = quartic_full_sample(threads)
+        log("host baselines of the coupled-2D, EVB-instanton and water workloads")
+        cpu_others = cpu_workloads_sample(threads)
 
     cpu = None
     if not args.no_cpu:
@@ -750,9 +751,9 @@ def main():
         "equal_splitting": eqs,
         "andersen": andersen,
         "instanton": inst,
-        "coupled2d": config2,
-        "config1_full": config1,
-        "cpu_configs": cpu_configs,
+        "coupled2d": coupled,
+        "quartic_full": quartic,
+        "cpu_other_workloads": cpu_others,
         "water": water,
     }
     print(json.dumps(out), flush=True)
