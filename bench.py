@@ -303,7 +303,8 @@ def instanton_sample(pes, sp, nchain=256, reps=10, seed=4):
     Throughput: bead-evaluations per second of U_M with its adjoint gradient and of the HVP,
     log-determinant bead blocks per second (lndetJ0 at well 1); each figure includes the
     host copies of its inputs/outputs (the gradient and HVP figures move n*ndof doubles each
-    way per call), except the energy-only U_M figure, which returns one double per chain.
+    way per call), except the energy-only U_M figure, which returns one double per chain,
+    and the *_device figures, whose vectors stay in device memory (rp_ring_*_device).
 
     Solution (rpi_ser.f90:95, 146-191, 221-237, 354-381): V0 = V(well1); every chain's
     instanton from a straight line by the device L-BFGS with the reference's settings
@@ -341,6 +342,28 @@ def instanton_sample(pes, sp, nchain=256, reps=10, seed=4):
     for _ in range(reps):
         ring.hvp(v)
     t_hvp = (time.perf_counter() - t0) / reps
+    # the same two calls with the chains resident in device memory (rp_ring_*_device, as the
+    # torch.autograd wrapper runs them): no host copies of full vectors
+    import torch
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    xd = torch.as_tensor(np.concatenate([c.reshape(-1) for c in chains]), device=dev)
+    gd = torch.empty_like(xd)
+    vd = torch.as_tensor(np.concatenate([c.reshape(-1) for c in v]), device=dev)
+    hd = torch.empty_like(vd)
+    torch.cuda.synchronize()
+    ring.set_x_device(xd.data_ptr())
+    ring.um_device(gd.data_ptr())
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ring.um_device(gd.data_ptr())
+    t_um_dev = (time.perf_counter() - t0) / reps
+    ring.hvp_device(vd.data_ptr(), hd.data_ptr())
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ring.hvp_device(vd.data_ptr(), hd.data_ptr())
+    t_hvp_dev = (time.perf_counter() - t0) / reps
+    del xd, gd, vd, hd
     wells = [np.repeat(sp.well1[:, None], n, axis=1) for n in ns]
     ring.set_x(wells)
     ring.logdet(singlewell=True, skip_lowest=False)
@@ -391,6 +414,8 @@ def instanton_sample(pes, sp, nchain=256, reps=10, seed=4):
             "um_grad_bead_evals_per_s": beads / t_um,
             "um_energy_only_bead_evals_per_s": beads / t_e,
             "hvp_bead_evals_per_s": beads / t_hvp,
+            "um_grad_device_bead_evals_per_s": beads / t_um_dev,
+            "hvp_device_bead_evals_per_s": beads / t_hvp_dev,
             "logdet_bead_blocks_per_s": beads / t_ld,
             "solution": {
                 "settings": "V0 = V(well1); L-BFGS m = 8, factr = 1e6, pgtol = 1e-5, no iteration cap",
@@ -611,6 +636,10 @@ def main():
               "is independent, so the tasks would be dealt out to the GPUs (each its own batch "
               "of whole polymers) with only the TI sums gathered at the end.", file=sys.stderr)
         sys.exit(2)
+
+    # torch first, as under pytest: the library then shares the HIP runtime torch loads, and
+    # the instanton sample can hand it device tensors
+    import torch  # noqa: F401
 
     import rptunnel
     from rptunnel import Batch, Pes, specs
