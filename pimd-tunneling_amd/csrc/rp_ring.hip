@@ -1254,6 +1254,26 @@ extern "C" int rp_ring_hessian_band(rp_ring* r, int chain, int singlewell, doubl
 }
 
 namespace rp {
+// Launch order of factorisation tasks (task t factors chain chain_of[t]): longest chain first.
+// A task's time is its chain's length, and the workgroups a launch holds beyond what the
+// device runs at once start in block order as slots free; with the long chains first the
+// short ones fill the tail, so a launch of several tasks per CU takes about as long as its
+// longest chain instead of up to twice that.
+static std::vector<int> longest_first(const rp_ring* r, const std::vector<int>& chain_of) {
+  std::vector<int> ord(chain_of.size());
+  for (size_t t = 0; t < ord.size(); ++t) ord[t] = (int)t;
+  std::stable_sort(ord.begin(), ord.end(),
+                   [&](int u, int v) { return r->n[chain_of[u]] > r->n[chain_of[v]]; });
+  return ord;
+}
+
+template <class V>
+static void permute(V& v, const std::vector<int>& ord) {
+  V w(v.size());
+  for (size_t t = 0; t < ord.size(); ++t) w[t] = v[ord[t]];
+  v.swap(w);
+}
+
 // Factor every task t (chain chain_of[t], or chain t when chain_of is null) at shift[t]:
 // log|det|, the number of negative pivots (eigenvalues below the shift) and, when keep, the
 // S_i^-1 of the chain for inverse iteration.
@@ -1326,6 +1346,7 @@ int ring_lowest_eigen(rp_ring* r, const std::vector<int>& ng, double* eta) {
       sh[c] = mu[c] * (1.0 - 1e-9);  // just off the estimate, so H - sh I stays factorable
       if (act[c]) map.push_back(c);
     }
+    permute(map, longest_first(r, map));
     std::vector<double> tsh;
     for (int c : map) tsh.push_back(sh[c]);
     if ((rc = d_map.upload(map.data(), map.size(), r->s)) ||
@@ -1366,6 +1387,9 @@ int ring_lowest_eigen(rp_ring* r, const std::vector<int>& ng, double* eta) {
       }
     }
     if (map.empty()) break;
+    const std::vector<int> ord = longest_first(r, map);
+    permute(map, ord);
+    permute(tsh, ord);
     std::vector<int> cnt(map.size());
     if ((rc = d_map.upload(map.data(), map.size(), r->s)) ||
         (rc = d_tld.upload(tsh.data(), tsh.size(), r->s)) ||
@@ -1399,6 +1423,9 @@ int ring_lowest_eigen(rp_ring* r, const std::vector<int>& ng, double* eta) {
     tsh.push_back(lo[c]);
   }
   if (!map.empty()) {
+    const std::vector<int> ord = longest_first(r, map);
+    permute(map, ord);
+    permute(tsh, ord);
     std::vector<double> e(C);
     if ((rc = d_map.upload(map.data(), map.size(), r->s)) ||
         (rc = d_tld.upload(tsh.data(), tsh.size(), r->s)) ||
@@ -1449,12 +1476,16 @@ static int ring_bisect(rp_ring* r, const std::vector<int>& tchain, const std::ve
   DevBuf<double> d_mid, d_ldt;
   DevBuf<int> d_map, d_cnt;
   auto cleanup = [&]() { d_mid.free(); d_ldt.free(); d_map.free(); d_cnt.free(); };
+  // launch slot u runs task ord[u]
+  const std::vector<int> ord = longest_first(r, tchain);
+  std::vector<int> lchain(tchain);
+  permute(lchain, ord);
   if ((rc = d_mid.alloc(nt)) || (rc = d_ldt.alloc(nt)) || (rc = d_map.alloc(nt)) ||
-      (rc = d_cnt.alloc(nt)) || (rc = d_map.upload(tchain.data(), nt, r->s))) {
+      (rc = d_cnt.alloc(nt)) || (rc = d_map.upload(lchain.data(), nt, r->s))) {
     cleanup();
     return rc;
   }
-  std::vector<double> mid(nt);
+  std::vector<double> mid(nt), lmid(nt);
   std::vector<int> k(nt);
   // 200 halvings take any bracket of doubles to adjacent values; the loop ends sooner, when
   // every bracket is within a few ulps
@@ -1465,7 +1496,8 @@ static int ring_bisect(rp_ring* r, const std::vector<int>& tchain, const std::ve
       open |= (b[t] - a[t]) > 4e-16 * std::max(std::fabs(a[t]), std::fabs(b[t]));
     }
     if (!open) break;
-    if ((rc = d_mid.upload(mid.data(), nt, r->s))) break;
+    for (int u = 0; u < nt; ++u) lmid[u] = mid[ord[u]];
+    if ((rc = d_mid.upload(lmid.data(), nt, r->s))) break;
     rc = with_logdet_kernel(r->D, [&](auto kern, int nth) -> int {
       kern<<<nt, nth, 0, r->s>>>(r->D, r->d_n.p, r->d_boff.p, r->d_betan.p, r->d_massd.p,
                                  r->d_blocks.p, d_mid.p, nullptr, d_ldt.p, d_cnt.p, d_map.p);
@@ -1477,7 +1509,10 @@ static int ring_bisect(rp_ring* r, const std::vector<int>& tchain, const std::ve
       rc = set_error(RP_ERR_HIP, "inertia count");
       break;
     }
-    for (int t = 0; t < nt; ++t) (k[t] > tj[t] ? b[t] : a[t]) = mid[t];
+    for (int u = 0; u < nt; ++u) {
+      const int t = ord[u];
+      (k[u] > tj[t] ? b[t] : a[t]) = mid[t];
+    }
   }
   for (int t = 0; t < nt; ++t) eig[t] = 0.5 * (a[t] + b[t]);
   cleanup();
