@@ -523,6 +523,7 @@ __global__ void __launch_bounds__(kRingThreads) ring_param_grad_kernel(RingArgs 
 // kept for inverse iteration on the eigenvalue nearest zero.
 namespace {
 constexpr int kMaxD = 56;  // ndof bound of the log-determinant workspace (LDS)
+constexpr int kPinP = 3;   // elements per thread whose step operands are read up front
 
 // One wave per chain.  Lane l owns the elements e = l + 64 q (q < P) of every D x D block, so
 // the Hessian block of the next bead is loaded into registers while the current one is
@@ -594,21 +595,47 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
     // eliminations two pivots at a time (a 2 x 2 pivot block, inverted explicitly, is two
     // successive Gauss-Jordan steps), so a bead takes ceil(D/2) barriers; the pivots of the
     // pair are a and det/a, the same two LDL^T pivots as one at a time
-    double mypiv = 1.0;
+    // thread k < D keeps pivot k as pnum / pden (the second pivot of a pair is det / a), divided
+    // once per bead instead of in the step, where one thread's division holds up the barrier
+    double pnum = 1.0, pden = 1.0;
     for (int k = 0; k < D; k += 2) {
       const double* A = Sb[cur];
       double* B = Sb[cur ^ 1];
       if (k + 1 < D) {
         const int l = k + 1;
+        // every element's operands are read before the pivot block's inverse, and the empty
+        // asm below needs them all: otherwise the compiler moves each read into the branch
+        // that uses it, and a step waits on one LDS round trip per element and operand.
+        // Pinned up to kPinP elements per thread: beyond that the held operands cost
+        // occupancy (P = 6 would go from 3 to 2 waves per SIMD)
+        double u0s[P], u1s[P], w0s[P], w1s[P], aes[P];
+        auto operands = [&](int q) {
+          const int r = er[q], cc = ec[q];
+          u0s[q] = A[r * D + k];
+          u1s[q] = A[r * D + l];
+          w0s[q] = A[k * D + cc];
+          w1s[q] = A[l * D + cc];
+          aes[q] = A[tid + T * q];
+        };
+        if constexpr (P <= kPinP) {
+#pragma unroll
+          for (int q = 0; q < P; ++q) operands(q);
+        }
         const double a = A[k * D + k], b = A[k * D + l], cq = A[l * D + k], d = A[l * D + l];
         const double det = fma(a, d, -b * cq), id = 1.0 / det;
         const double p00 = d * id, p01 = -b * id, p10 = -cq * id, p11 = a * id;  // block^-1
-        mypiv = tid == k ? a : tid == l ? det / a : mypiv;
+        pnum = tid == k ? a : tid == l ? det : pnum;
+        pden = tid == k ? 1.0 : tid == l ? a : pden;
+        if constexpr (P <= kPinP) {
+#pragma unroll
+          for (int q = 0; q < P; ++q)
+            asm volatile("" ::"v"(u0s[q]), "v"(u1s[q]), "v"(w0s[q]), "v"(w1s[q]), "v"(aes[q]));
+        }
 #pragma unroll
         for (int q = 0; q < P; ++q) {
+          if constexpr (P > kPinP) operands(q);
           const int e = tid + T * q, r = er[q], cc = ec[q];
-          const double u0 = A[r * D + k], u1 = A[r * D + l], w0 = A[k * D + cc], w1 = A[l * D + cc];
-          const double ae = A[e];
+          const double u0 = u0s[q], u1 = u1s[q], w0 = w0s[q], w1 = w1s[q], ae = aes[q];
           const double pw0 = p00 * w0 + p01 * w1, pw1 = p10 * w0 + p11 * w1;  // block^-1 A[K][cc]
           const double up0 = u0 * p00 + u1 * p10, up1 = u0 * p01 + u1 * p11;  // A[r][K] block^-1
           const bool rk = r == k || r == l, ck = cc == k || cc == l;
@@ -620,12 +647,28 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
           B[e] = v;
         }
       } else {
+        double mrks[P], mkcs[P], aes[P];  // read first, as in the pair step
+        auto operands = [&](int q) {
+          mrks[q] = A[er[q] * D + k];
+          mkcs[q] = A[k * D + ec[q]];
+          aes[q] = A[tid + T * q];
+        };
+        if constexpr (P <= kPinP) {
+#pragma unroll
+          for (int q = 0; q < P; ++q) operands(q);
+        }
         const double p = A[k * D + k], ip = 1.0 / p;
-        mypiv = tid == k ? p : mypiv;
+        pnum = tid == k ? p : pnum;
+        pden = tid == k ? 1.0 : pden;
+        if constexpr (P <= kPinP) {
+#pragma unroll
+          for (int q = 0; q < P; ++q) asm volatile("" ::"v"(mrks[q]), "v"(mkcs[q]), "v"(aes[q]));
+        }
 #pragma unroll
         for (int q = 0; q < P; ++q) {
+          if constexpr (P > kPinP) operands(q);
           const int e = tid + T * q, r = er[q], cc = ec[q];
-          const double mrk = A[r * D + k], mkc = A[k * D + cc], a = A[e];
+          const double mrk = mrks[q], mkc = mkcs[q], a = aes[q];
           const double vr = cc == k ? ip : mkc * ip;
           const double vo = cc == k ? -mrk * ip : a - mrk * mkc * ip;
           B[e] = r == k ? vr : vo;
@@ -636,6 +679,7 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
     }
     // pivots: log|p| and the sign count; threads k < D <= 56 of wave 0 hold pivot k
     if (tid < 64) {
+      const double mypiv = pnum / pden;
       double l = tid < D ? log(fabs(mypiv)) : 0.0;
       int ng = (tid < D && mypiv < 0.0) ? 1 : 0;
       for (int o = 32; o > 0; o >>= 1) {
