@@ -539,7 +539,14 @@ constexpr int kPinP = 3;   // elements per thread whose step operands are read u
 // of eliminations): the pivot of elimination k is the k-th pivot of the matrix's LDL^T
 // factorisation (a ratio of leading principal minors), so their logs sum to log|det| and
 // their signs give the inertia (Sylvester).
-template <int P, int T>
+//
+// SYM: the lower triangle only.  The Gauss-Jordan iterates keep M[j][i] = s M[i][j], with
+// s = -1 exactly when one of i, j has been swept (a pivot already eliminated) and +1
+// otherwise, so thread t owns lower element t (r >= cc, row-major) and writes both it and its
+// mirror: P elements per thread cover D (D + 1) / 2 instead of D^2, the reads are the same,
+// and the LDS tile (PT elements per thread) still holds the whole matrix.  Threads beyond the
+// triangle own the pad element (D, D).
+template <int P, int T, bool SYM = false, int PT = P>
 __global__ void __launch_bounds__(T) ring_logdet_kernel(
     int D, const int* __restrict__ nn, const int64_t* __restrict__ boff,
     const double* __restrict__ betan, const double* __restrict__ massd,
@@ -549,7 +556,7 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
   // it (and k D + cc < D^2 <= T P); the elements beyond D*D hold values nobody reads back,
   // which keeps every load and store unconditional (no branch, so the loads of the next
   // block stay in flight).  Sized by T P, so several workgroups share a CU's LDS.
-  constexpr int kTile = T * P + kMaxD;
+  constexpr int kTile = T * PT + kMaxD;
   __shared__ double Sb[2][kTile];
   // one task per workgroup: chain c (task t of a chain_of map, else chain t) factored at
   // the shift of task t; outputs per task
@@ -557,16 +564,26 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
   const int n = nn[c], tid = threadIdx.x, DD = D * D;
   const double bn2 = betan[c] * betan[c], c2 = 1.0 / (bn2 * bn2);
   const double sig = shift ? shift[t] : 0.0;  // factor H - sig I
-  int er[P], ec[P], eg[P];
+  int er[P], ec[P], eg[P], ei[P], mi[P];  // row, column, global index, LDS index, mirror
   double wgt[P], diag[P], nxt[P];
 #pragma unroll
   for (int q = 0; q < P; ++q) {
     const int e = tid + T * q;
-    eg[q] = min(e, DD - 1);  // clamped index of the global block loads
-    er[q] = e / D;
-    ec[q] = e - er[q] * D;
-    const int rm = min(er[q], D - 1);
-    wgt[q] = 1.0 / sqrt(massd[rm] * massd[ec[q]]);
+    if constexpr (SYM) {
+      int r = 0;
+      while ((r + 1) * (r + 2) / 2 <= e && r < D) ++r;  // row of lower element e
+      const bool pad = r >= D;
+      er[q] = pad ? D : r;
+      ec[q] = pad ? D : e - r * (r + 1) / 2;
+    } else {
+      er[q] = e / D;
+      ec[q] = e - er[q] * D;
+    }
+    ei[q] = SYM ? er[q] * D + ec[q] : e;
+    mi[q] = ec[q] * D + er[q];
+    eg[q] = min(ei[q], DD - 1);  // clamped index of the global block loads
+    const int rm = min(er[q], D - 1), cm = min(ec[q], D - 1);
+    wgt[q] = 1.0 / sqrt(massd[rm] * massd[cm]);
     diag[q] = er[q] == ec[q] ? 2.0 / bn2 - sig : 0.0;
   }
   const double* base = blocks + (size_t)boff[c] * DD;
@@ -581,10 +598,12 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
     double* M = Sb[cur];
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-      const int e = tid + T * q;
+      const int e = ei[q];
       const double v = nxt[q] * wgt[q] + diag[q];
       const double prev = M[e];
-      M[e] = coupled ? v - c2 * prev : v;
+      const double val = coupled ? v - c2 * prev : v;
+      M[e] = val;
+      if constexpr (SYM) M[mi[q]] = val;  // A_i and S_{i-1}^-1 are symmetric
     }
     // the next bead's block (a valid address for the last bead too), in flight during the
     // elimination
@@ -615,7 +634,7 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
           u1s[q] = A[r * D + l];
           w0s[q] = A[k * D + cc];
           w1s[q] = A[l * D + cc];
-          aes[q] = A[tid + T * q];
+          aes[q] = A[ei[q]];
         };
         if constexpr (P <= kPinP) {
 #pragma unroll
@@ -634,7 +653,7 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
 #pragma unroll
         for (int q = 0; q < P; ++q) {
           if constexpr (P > kPinP) operands(q);
-          const int e = tid + T * q, r = er[q], cc = ec[q];
+          const int e = ei[q], r = er[q], cc = ec[q];
           const double u0 = u0s[q], u1 = u1s[q], w0 = w0s[q], w1 = w1s[q], ae = aes[q];
           const double pw0 = p00 * w0 + p01 * w1, pw1 = p10 * w0 + p11 * w1;  // block^-1 A[K][cc]
           const double up0 = u0 * p00 + u1 * p10, up1 = u0 * p01 + u1 * p11;  // A[r][K] block^-1
@@ -645,13 +664,14 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
                        : -(cc == k ? up0 : up1))
                  : v;
           B[e] = v;
+          if constexpr (SYM) B[mi[q]] = (r <= l) != (cc <= l) ? -v : v;  // swept: 0..l
         }
       } else {
         double mrks[P], mkcs[P], aes[P];  // read first, as in the pair step
         auto operands = [&](int q) {
           mrks[q] = A[er[q] * D + k];
           mkcs[q] = A[k * D + ec[q]];
-          aes[q] = A[tid + T * q];
+          aes[q] = A[ei[q]];
         };
         if constexpr (P <= kPinP) {
 #pragma unroll
@@ -667,11 +687,13 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
 #pragma unroll
         for (int q = 0; q < P; ++q) {
           if constexpr (P > kPinP) operands(q);
-          const int e = tid + T * q, r = er[q], cc = ec[q];
+          const int e = ei[q], r = er[q], cc = ec[q];
           const double mrk = mrks[q], mkc = mkcs[q], a = aes[q];
           const double vr = cc == k ? ip : mkc * ip;
           const double vo = cc == k ? -mrk * ip : a - mrk * mkc * ip;
-          B[e] = r == k ? vr : vo;
+          const double v = r == k ? vr : vo;
+          B[e] = v;
+          if constexpr (SYM) B[mi[q]] = (r <= k) != (cc <= k) ? -v : v;  // swept: 0..k
         }
       }
       cur ^= 1;
@@ -692,9 +714,20 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
     if (sinv) {  // kept for inverse iteration; inertia-only calls pass null
       double* out = sinv + (size_t)(boff[c] + i) * DD;
       const double* Minv = Sb[cur];
+      // each thread stores the elements it overwrites first in the next bead (no barrier
+      // between): under SYM its lower element and the mirror, else its own elements
+      if constexpr (SYM) {
 #pragma unroll
-      for (int q = 0; q < P; ++q)
-        if (tid + T * q < DD) out[tid + T * q] = Minv[tid + T * q];
+        for (int q = 0; q < P; ++q)
+          if (ei[q] < DD) {
+            out[ei[q]] = Minv[ei[q]];
+            out[mi[q]] = Minv[mi[q]];
+          }
+      } else {
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+          if (tid + T * q < DD) out[tid + T * q] = Minv[tid + T * q];
+      }
     }
   }
   if (tid == 0) {
@@ -748,7 +781,7 @@ template <typename F>
 int with_logdet_kernel(int D, F&& f) {
   if (D * D <= 64) return f(ring_logdet_kernel<1, 64>, 64);
   if (D * D <= 256) return f(ring_logdet_kernel<1, 256>, 256);
-  if (D * D <= 768) return f(ring_logdet_kernel<3, 256>, 256);
+  if (D * D <= 768) return f(ring_logdet_kernel<2, 256, true, 3>, 256);  // D (D + 1) / 2 <= 512
   if (D * D <= 1536) return f(ring_logdet_kernel<6, 256>, 256);
   return f(ring_logdet_kernel<13, 256>, 256);
 }
