@@ -782,8 +782,8 @@ int with_logdet_kernel(int D, F&& f) {
   if (D * D <= 64) return f(ring_logdet_kernel<1, 64>, 64);
   if (D * D <= 256) return f(ring_logdet_kernel<1, 256>, 256);
   if (D * D <= 768) return f(ring_logdet_kernel<2, 256, true, 3>, 256);  // D (D + 1) / 2 <= 512
-  if (D * D <= 1536) return f(ring_logdet_kernel<6, 256>, 256);
-  return f(ring_logdet_kernel<13, 256>, 256);
+  if (D * D <= 1536) return f(ring_logdet_kernel<4, 256, true, 6>, 256);  // D (D + 1) / 2 <= 1024
+  return f(ring_logdet_kernel<7, 256, true, 13>, 256);  // D (D + 1) / 2 <= 1792
 }
 
 // The block solve y = H^-1 v through the stored factors S_i^-1 (the off-diagonal blocks are

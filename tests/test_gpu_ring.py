@@ -147,6 +147,59 @@ def test_lowest_eigen_and_logdet_harmonic_against_eigvalsh(D):
     ring.close()
 
 
+def _dense_host_pes(D, seed):
+    """A host potential with dense D x D Hessians (no device twin has 28 <= D <= 39):
+    V = (x - x0)^T K (x - x0) / 2 + 0.1 sum (x - x0)^4, K = QQ^T random, positive definite."""
+    rng = np.random.default_rng(seed)
+    Q = rng.standard_normal((D, D))
+    K = 0.05 * (Q @ Q.T) / D + 0.01 * np.eye(D)
+    x0 = rng.standard_normal(D)
+
+    def potforce(x):
+        d = x - x0[:, None]
+        return K @ d + 0.4 * d ** 3, 0.5 * np.sum(d * (K @ d), axis=0) + 0.1 * np.sum(d ** 4, axis=0)
+
+    def hessian(x):
+        d = x - x0[:, None]
+        h = np.repeat(K[None], x.shape[1], axis=0)
+        h[:, np.arange(D), np.arange(D)] += 1.2 * d.T ** 2
+        return h
+
+    return Pes.host(3, D // 3, potforce, hessian), x0, np.full(D // 3, 1836.0)
+
+
+@pytest.mark.parametrize("case", ["water2", "dense36", "water6"])
+def test_logdet_dense_blocks_against_eigvalsh(case):
+    """The block factorisation with dense D x D blocks (D = 18, 36 and 54 dofs, one per
+    block-size class of the lower-triangle update: the 2- and 6-molecule water clusters on
+    the device, a dense quartic host potential for 36) on chains near the minimum, against
+    eigvalsh of the library's own band (rp_ring_hessian_band, the blocks the factorisation
+    reads): the negative count, ln over the positive eigenvalues (rpi_ser.f90:227-236) 1e-9
+    relative, and eta_1 when none is negative, 1e-8 relative."""
+    rng = np.random.default_rng(len(case))
+    if case == "dense36":
+        pes, x0, mass = _dense_host_pes(36, 11)
+        a = b = x0
+        scale = 0.3
+    else:
+        sp = specs.water_cluster(nmol=int(case[-1]))
+        pes, x0, mass, a, b = Pes(sp), sp.well1, sp.mass, sp.well1, sp.well2
+        scale = 0.02
+    D = pes.ndof
+    ns, betan = [5, 23], [30.0, 18.0]
+    xs = [x0[:, None] + scale * rng.standard_normal((D, n)) for n in ns]
+    ring = Ring(pes, mass, ns, betan, a, b)
+    ring.set_x(xs)
+    ld0, nn0, eta0 = ring.logdet(singlewell=False, skip_lowest=False, want_eta=True)
+    for k, n in enumerate(ns):
+        ev = O.detj_eigs(ring.UMhessian(k, singlewell=False))
+        assert nn0[k] == np.sum(ev < 0), (n, nn0[k], ev[:4])
+        assert abs(ld0[k] - O.lndetj0(ev)) <= 1e-9 * abs(O.lndetj0(ev)), (n, ld0[k])
+        if nn0[k] == 0:
+            assert abs(eta0[k] - ev[0]) <= 1e-8 * ev[0], (n, eta0[k], ev[:3])
+    ring.close()
+
+
 def _linear_path(sp, n):
     k = np.arange(1, n + 1)
     return (np.outer(sp.well1, n - k) + np.outer(sp.well2, k - 1)) / (n - 1)
