@@ -654,15 +654,18 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
         for (int q = 0; q < P; ++q) {
           if constexpr (P > kPinP) operands(q);
           const int e = ei[q], r = er[q], cc = ec[q];
-          const double u0 = u0s[q], u1 = u1s[q], w0 = w0s[q], w1 = w1s[q], ae = aes[q];
-          const double pw0 = p00 * w0 + p01 * w1, pw1 = p10 * w0 + p11 * w1;  // block^-1 A[K][cc]
-          const double up0 = u0 * p00 + u1 * p10, up1 = u0 * p01 + u1 * p11;  // A[r][K] block^-1
-          const bool rk = r == k || r == l, ck = cc == k || cc == l;
-          double v = ae - (u0 * pw0 + u1 * pw1);
-          v = rk ? (r == k ? pw0 : pw1) : v;
-          v = ck ? (rk ? (r == k ? (cc == k ? p00 : p01) : (cc == k ? p10 : p11))
-                       : -(cc == k ? up0 : up1))
-                 : v;
+          // every case is a' - u~ P w~ with P = block^-1: a pivot row (r in K) takes
+          // u~ = -e_r (row r of P A[K][cc]), a pivot column w~ = e_cc (-A[r][K] P, or P
+          // itself for both), and a' = 0 for either; elsewhere u~, w~, a' are the element's
+          // row, column and value, A[r][cc] - A[r][K] P A[K][cc]
+          const bool rk = (r >> 1) == (k >> 1), ck = (cc >> 1) == (k >> 1);  // k even
+          const double u0 = rk ? ((r & 1) ? 0.0 : -1.0) : u0s[q];
+          const double u1 = rk ? ((r & 1) ? -1.0 : 0.0) : u1s[q];
+          const double w0 = ck ? ((cc & 1) ? 0.0 : 1.0) : w0s[q];
+          const double w1 = ck ? ((cc & 1) ? 1.0 : 0.0) : w1s[q];
+          const double ae = (rk || ck) ? 0.0 : aes[q];
+          const double pw0 = p00 * w0 + p01 * w1, pw1 = p10 * w0 + p11 * w1;  // P A[K][cc]
+          const double v = ae - (u0 * pw0 + u1 * pw1);
           B[e] = v;
           if constexpr (SYM) B[mi[q]] = (r <= l) != (cc <= l) ? -v : v;  // swept: 0..l
         }
