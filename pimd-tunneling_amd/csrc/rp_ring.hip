@@ -525,6 +525,16 @@ namespace {
 constexpr int kMaxD = 56;  // ndof bound of the log-determinant workspace (LDS)
 constexpr int kPinP = 3;   // elements per thread whose step operands are read up front
 
+// 1/x of a pivot: the hardware reciprocal estimate (about 26 correct bits) refined by two
+// Newton-Raphson steps r <- r + r (1 - x r), each doubling the correct bits.  The full
+// division adds scaling for denormal and near-overflow operands, which the pivots of these
+// blocks are not, and its extra steps lie on the elimination's critical path.
+__device__ __forceinline__ double pivot_recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
 // One wave per chain.  Lane l owns the elements e = l + 64 q (q < P) of every D x D block, so
 // the Hessian block of the next bead is loaded into registers while the current one is
 // eliminated, and the Schur update uses the previous inverse where it sits in LDS.  The
@@ -641,10 +651,12 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
           for (int q = 0; q < P; ++q) operands(q);
         }
         const double a = A[k * D + k], b = A[k * D + l], cq = A[l * D + k], d = A[l * D + l];
-        const double det = fma(a, d, -b * cq), id = 1.0 / det;
+        const double det = fma(a, d, -b * cq), id = pivot_recip(det);
         const double p00 = d * id, p01 = -b * id, p10 = -cq * id, p11 = a * id;  // block^-1
-        pnum = tid == k ? a : tid == l ? det : pnum;
-        pden = tid == k ? 1.0 : tid == l ? a : pden;
+        if (tid < 64) {  // pivot lanes are in wave 0 (D <= kMaxD < 64)
+          pnum = tid == k ? a : tid == l ? det : pnum;
+          pden = tid == k ? 1.0 : tid == l ? a : pden;
+        }
         if constexpr (P <= kPinP) {
 #pragma unroll
           for (int q = 0; q < P; ++q)
@@ -680,9 +692,11 @@ __global__ void __launch_bounds__(T) ring_logdet_kernel(
 #pragma unroll
           for (int q = 0; q < P; ++q) operands(q);
         }
-        const double p = A[k * D + k], ip = 1.0 / p;
-        pnum = tid == k ? p : pnum;
-        pden = tid == k ? 1.0 : pden;
+        const double p = A[k * D + k], ip = pivot_recip(p);
+        if (tid < 64) {
+          pnum = tid == k ? p : pnum;
+          pden = tid == k ? 1.0 : pden;
+        }
         if constexpr (P <= kPinP) {
 #pragma unroll
           for (int q = 0; q < P; ++q) asm volatile("" ::"v"(mrks[q]), "v"(mkcs[q]), "v"(aes[q]));
